@@ -1,0 +1,763 @@
+// Voxel-driven FDK backprojection for gfx950 (CDNA4), 32 views per pass.
+//
+// Reference behaviour (peng-ac/fdk):
+//   cudaFdkBP + SUBPIXEL      cudaBP/backprojectionFDK.cu:15-129
+//   BackProjectionFDK         cudaBP/backprojectionFDK.cu:161-302
+//   kernel_fdk_3Dgrid         cudaBP/rtkCudaFDKBackProjectionImageFilter.cu:156-191
+//   BackProjectionRTK         cudaBP/rtkCudaFDKBackProjectionImageFilter.cu:200-400
+//
+// For a voxel (x, y, z) and view s the 3x4 matrix M_s gives (u w, v w, w);
+// the sample is the bilinear value of the filtered view at (u, v), zero unless
+// 0 <= u < nu-1 and 0 <= v < nv-1, weighted by 1/w^2 and summed over views.
+//
+// MI355X design.
+//  * One wave64 owns one voxel column (x, y) and a run of 64*ZPT voxels along z
+//    (lane = z), plus their mirror voxels nz-1-z when the sub-volume has a
+//    mirror slab.  For the circular-orbit matrices rows 0 and 2 do not depend on
+//    z, so u, w and the u-direction bilinear weights are per (column, view)
+//    values, and v is affine in z: each voxel update is one FMA for v, two 8-byte
+//    gathers (rows u and u+1 of the transposed view, taps v and v+1) and five
+//    FMAs.  Lanes along z read one contiguous strip of each detector row, so a
+//    wave's gathers touch only a few 128-byte lines.
+//  * Matrices (and per-view scale/offset for FP16/BF16 storage) travel in the
+//    kernel argument block: uniform scalar loads, nothing global per handle, so
+//    two handles on one device never share state.
+//  * Volume in HBM is [x][y][zz] with zz fastest (zz = top slab then bottom
+//    slab), so the once-per-pass read-modify-write of a wave is one coalesced
+//    256-byte run.  GetVolumeData transposes to [zz][y][x] on the way out.
+//  * Blocks (4 waves = 2x2 columns) walk 32x32-column tiles, tiles walk the
+//    plane, and block ids are remapped so that each of the 8 XCDs works on a
+//    contiguous run of tiles: neighbouring columns share detector rows in the
+//    same L2.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr int kMaxProj = FDK_BP_MAX_PROJ;
+constexpr int kWave = 64;
+constexpr int kTile = 16;  // tile edge in 2x2-column patches (32 columns)
+
+struct BpMats {
+  float m[kMaxProj][12];
+};
+struct BpScale {
+  float so[kMaxProj][2];  // p = q * so[0] + so[1]
+};
+
+struct BpArgs {
+  float* vol;            // [xs][ys][zz], zz fastest
+  const void* proj;      // views, transposed: H rows (u) of W samples (v)
+  long long view_stride; // elements between views
+  int W, H;              // stored view width (v extent) and height (u extent)
+  int nproj;
+  int x0, y0, z0;        // global index of the sub-volume origin (top slab)
+  int nxs, nys;          // sub-volume extent in x and y
+  int zext;              // z extent handled per column: top depth (mirror) or nzz
+  int nzz;               // total zz extent of the buffer
+  int nz;                // global nz (mirror voxel z = nz - 1 - z)
+  int tiles_x;           // tiles along x
+};
+
+typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+template <int DT>
+struct Taps;
+
+template <>
+struct Taps<FDK_F32> {
+  // Taps (v, v+1) of rows u and u+1 as two unaligned 8-byte loads.
+  __device__ static inline void load(const void* base, long long off, int W, float& a0, float& a1,
+                                     float& b0, float& b1) {
+    const float* p = static_cast<const float*>(base) + off;
+    const f32x2_a4 a = *reinterpret_cast<const f32x2_a4*>(p);
+    const f32x2_a4 b = *reinterpret_cast<const f32x2_a4*>(p + W);
+    a0 = a.x; a1 = a.y; b0 = b.x; b1 = b.y;
+  }
+};
+
+template <>
+struct Taps<FDK_F16> {
+  __device__ static inline void load(const void* base, long long off, int W, float& a0, float& a1,
+                                     float& b0, float& b1) {
+    const __half* p = static_cast<const __half*>(base) + off;
+    a0 = __half2float(p[0]); a1 = __half2float(p[1]);
+    b0 = __half2float(p[W]); b1 = __half2float(p[W + 1]);
+  }
+};
+
+template <>
+struct Taps<FDK_BF16> {
+  __device__ static inline float bf(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+  __device__ static inline void load(const void* base, long long off, int W, float& a0, float& a1,
+                                     float& b0, float& b1) {
+    const unsigned short* p = static_cast<const unsigned short*>(base) + off;
+    a0 = bf(p[0]); a1 = bf(p[1]); b0 = bf(p[W]); b1 = bf(p[W + 1]);
+  }
+};
+
+// One gathered, weighted bilinear sample at (u fixed per row pair, v):
+// returns w0 * lerp(row u) + w1 * lerp(row u+1), or 0 outside 0 <= v < W-1.
+template <int DT>
+__device__ inline float sample_v(const void* rows, float v, int W, float w0, float w1) {
+  const float fv = floorf(v);
+  const int iv = (int)fv;
+  const bool ok = (unsigned)iv <= (unsigned)(W - 2);  // 0 <= v < W-1
+  const float t = v - fv;
+  float a0, a1, b0, b1;
+  Taps<DT>::load(rows, ok ? iv : 0, W, a0, a1, b0, b1);
+  const float ra = fmaf(t, a1 - a0, a0);
+  const float rb = fmaf(t, b1 - b0, b0);
+  const float s = fmaf(w0, ra, w1 * rb);
+  return ok ? s : 0.0f;
+}
+
+// Block id remap so that consecutive logical blocks land on one XCD: hardware
+// hands block b to XCD b % 8, logical block = (b % 8) * (n / 8) + b / 8.
+__device__ inline unsigned xcd_remap(unsigned b, unsigned n) {
+  return (n % 8u == 0u) ? (b % 8u) * (n / 8u) + b / 8u : b;
+}
+
+template <int DT, bool SYM, int ZPT>
+__global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpScale S) {
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int tile = lb / (kTile * kTile);
+  const int within = lb % (kTile * kTile);
+  const int px = (tile % a.tiles_x) * kTile + within % kTile;
+  const int py = (tile / a.tiles_x) * kTile + within / kTile;
+  const int xs = 2 * px + (wave & 1);
+  const int ys = 2 * py + (wave >> 1);
+  if (xs >= a.nxs || ys >= a.nys) return;  // whole wave leaves together
+  const int zbase = blockIdx.y * (kWave * ZPT) + lane;
+  if (zbase >= a.zext) return;
+
+  const float x = (float)(a.x0 + xs);
+  const float y = (float)(a.y0 + ys);
+  float acc[ZPT], accm[ZPT];
+  float zt[ZPT], zm[ZPT];
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    acc[j] = 0.f;
+    accm[j] = 0.f;
+    const int zl = zbase + j * kWave;
+    zt[j] = (float)(a.z0 + zl);
+    zm[j] = (float)(a.nz - 1 - (a.z0 + zl));
+  }
+
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    const float wd = fmaf(m[8], x, fmaf(m[9], y, m[11]));
+    const float Z = 1.0f / wd;
+    const float U = fmaf(m[0], x, fmaf(m[1], y, m[3])) * Z;
+    const float Vb = fmaf(m[4], x, fmaf(m[5], y, m[7])) * Z;
+    const float dV = m[6] * Z;
+    const float Z2 = Z * Z;
+    const float fu0 = floorf(U);
+    const int iu = (int)fu0;
+    const bool uok = (unsigned)iu <= (unsigned)(a.H - 2);  // 0 <= u < H-1
+    const float fu = U - fu0;
+    float w0 = uok ? (1.0f - fu) * Z2 : 0.0f;
+    float w1 = uok ? fu * Z2 : 0.0f;
+    float off = 0.0f;
+    if (DT != FDK_F32) {
+      w0 *= S.so[s][0];
+      w1 *= S.so[s][0];
+      off = S.so[s][1] * Z2;  // bilinear weights sum to 1 inside the view
+    }
+    const long long rowoff = (long long)s * a.view_stride + (long long)(uok ? iu : 0) * a.W;
+    const void* rows;
+    if (DT == FDK_F32) rows = static_cast<const float*>(a.proj) + rowoff;
+    else rows = static_cast<const unsigned short*>(a.proj) + rowoff;
+#pragma unroll
+    for (int j = 0; j < ZPT; ++j) {
+      const float v = fmaf(zt[j], dV, Vb);
+      float c = sample_v<DT>(rows, v, a.W, w0, w1);
+      if (DT != FDK_F32) c += (uok && v >= 0.f && v < (float)(a.W - 1)) ? off : 0.f;
+      acc[j] += c;
+      if (SYM) {
+        const float vm = fmaf(zm[j], dV, Vb);
+        float cm = sample_v<DT>(rows, vm, a.W, w0, w1);
+        if (DT != FDK_F32) cm += (uok && vm >= 0.f && vm < (float)(a.W - 1)) ? off : 0.f;
+        accm[j] += cm;
+      }
+    }
+  }
+
+  float* col = a.vol + ((long long)xs * a.nys + ys) * (long long)a.nzz;
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    const int zl = zbase + j * kWave;
+    if (zl < a.zext) {
+      col[zl] += acc[j];
+      if (SYM) col[a.nzz - 1 - zl] += accm[j];
+    }
+  }
+}
+
+// General matrices (a z term in rows 0 or 2): every voxel does the full
+// projective map.  Same layout and sampling as bp_column_kernel, ZPT = 1.
+template <int DT, bool SYM>
+__global__ __launch_bounds__(256) void bp_general_kernel(BpArgs a, BpMats M, BpScale S) {
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int tile = lb / (kTile * kTile);
+  const int within = lb % (kTile * kTile);
+  const int px = (tile % a.tiles_x) * kTile + within % kTile;
+  const int py = (tile / a.tiles_x) * kTile + within / kTile;
+  const int xs = 2 * px + (wave & 1);
+  const int ys = 2 * py + (wave >> 1);
+  if (xs >= a.nxs || ys >= a.nys) return;
+  const int zl = blockIdx.y * kWave + lane;
+  if (zl >= a.zext) return;
+  const float x = (float)(a.x0 + xs), y = (float)(a.y0 + ys);
+  const float zs[2] = {(float)(a.z0 + zl), (float)(a.nz - 1 - (a.z0 + zl))};
+  float acc[2] = {0.f, 0.f};
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    for (int q = 0; q < (SYM ? 2 : 1); ++q) {
+      const float z = zs[q];
+      const float Z = 1.0f / fmaf(m[8], x, fmaf(m[9], y, fmaf(m[10], z, m[11])));
+      const float U = fmaf(m[0], x, fmaf(m[1], y, fmaf(m[2], z, m[3]))) * Z;
+      const float V = fmaf(m[4], x, fmaf(m[5], y, fmaf(m[6], z, m[7]))) * Z;
+      const float fu0 = floorf(U);
+      const int iu = (int)fu0;
+      const bool uok = (unsigned)iu <= (unsigned)(a.H - 2);
+      const float fu = U - fu0, Z2 = Z * Z;
+      float w0 = uok ? (1.f - fu) * Z2 : 0.f, w1 = uok ? fu * Z2 : 0.f;
+      if (DT != FDK_F32) { w0 *= S.so[s][0]; w1 *= S.so[s][0]; }
+      const long long rowoff = (long long)s * a.view_stride + (long long)(uok ? iu : 0) * a.W;
+      const void* rows = (DT == FDK_F32) ? (const void*)(static_cast<const float*>(a.proj) + rowoff)
+                                         : (const void*)(static_cast<const unsigned short*>(a.proj) + rowoff);
+      float c = sample_v<DT>(rows, V, a.W, w0, w1);
+      if (DT != FDK_F32) c += (uok && V >= 0.f && V < (float)(a.W - 1)) ? S.so[s][1] * Z2 : 0.f;
+      acc[q] += c;
+    }
+  }
+  float* col = a.vol + ((long long)xs * a.nys + ys) * (long long)a.nzz;
+  col[zl] += acc[0];
+  if (SYM) col[a.nzz - 1 - zl] += acc[1];
+}
+
+// [n][rows][cols] -> [n][cols][rows] through a 32x33 LDS tile.
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ src,
+                                                        float* __restrict__ dst, int rows, int cols,
+                                                        long long src_stride, long long dst_stride) {
+  __shared__ float t[32][33];
+  const int n = blockIdx.z;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const float* s = src + n * src_stride;
+  float* d = dst + n * dst_stride;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + threadIdx.x;
+    if (r < rows && c < cols) t[k][threadIdx.x] = s[(long long)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + threadIdx.x;
+    if (r < rows && c < cols) d[(long long)c * rows + r] = t[threadIdx.x][k];
+  }
+}
+
+int launch_transpose(const float* src, float* dst, int rows, int cols, int n, long long ss,
+                     long long ds, hipStream_t st) {
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32, n);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, src, dst, rows, cols, ss, ds);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+// Volume [xs][ys][zz] -> [zz][ys][xs] for zz in [z_begin, z_begin + nzc).
+__global__ __launch_bounds__(256) void volume_out_kernel(const float* __restrict__ vol,
+                                                         float* __restrict__ out, int nxs, int nys,
+                                                         int nzz, int z_begin, int nzc) {
+  __shared__ float t[32][33];
+  const int ys = blockIdx.z;
+  const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int xs = x0 + k, zc = z0 + threadIdx.x;
+    if (xs < nxs && zc < nzc)
+      t[k][threadIdx.x] = vol[((long long)xs * nys + ys) * nzz + z_begin + zc];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int zc = z0 + k, xs = x0 + threadIdx.x;
+    if (xs < nxs && zc < nzc) out[((long long)zc * nys + ys) * nxs + xs] = t[threadIdx.x][k];
+  }
+}
+
+struct Box {
+  int64_t x, y, z, w, h, d;
+};
+
+}  // namespace
+
+struct fdk_bp {
+  int proj_w = 0, proj_h = 0, proj_count = 0;
+  int nx = 0, ny = 0, nz = 0;
+  Box top{}, bot{};
+  int nzz = 0;
+  bool sym = false;
+  float* vol = nullptr;
+  float* staging = nullptr;  // proj_count views, host uploads land here
+  float* staging_t = nullptr;
+  size_t staging_elems = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int count = 0;
+  float kernel_ms = 0.f;
+  int device = 0;
+};
+
+namespace {
+
+int fill_args(fdk_bp* h, int W, int H, long long view_stride, int nproj, const void* proj,
+              BpArgs& a) {
+  a.vol = h->vol;
+  a.proj = proj;
+  a.view_stride = view_stride;
+  a.W = W;
+  a.H = H;
+  a.nproj = nproj;
+  a.x0 = (int)h->top.x;
+  a.y0 = (int)h->top.y;
+  a.z0 = (int)h->top.z;
+  a.nxs = (int)h->top.w;
+  a.nys = (int)h->top.h;
+  a.zext = h->sym ? (int)h->top.d : h->nzz;
+  a.nzz = h->nzz;
+  a.nz = h->nz;
+  const int npx = (a.nxs + 1) / 2, npy = (a.nys + 1) / 2;
+  a.tiles_x = (npx + kTile - 1) / kTile;
+  (void)npy;
+  return FDK_OK;
+}
+
+template <int DT>
+int launch_bp_dt(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, bool general) {
+  const int npy = (a.nys + 1) / 2;
+  const int tiles_y = (npy + kTile - 1) / kTile;
+  const unsigned nblk = (unsigned)(a.tiles_x * tiles_y * kTile * kTile);
+  constexpr int ZPT = 4;
+  if (general) {
+    dim3 grid(nblk, (a.zext + kWave - 1) / kWave);
+    if (h->sym) hipLaunchKernelGGL((bp_general_kernel<DT, true>), grid, dim3(256), 0, h->stream, a, M, S);
+    else hipLaunchKernelGGL((bp_general_kernel<DT, false>), grid, dim3(256), 0, h->stream, a, M, S);
+  } else {
+    dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
+    if (h->sym) hipLaunchKernelGGL((bp_column_kernel<DT, true, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
+    else hipLaunchKernelGGL((bp_column_kernel<DT, false, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
+  }
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+int launch_bp(fdk_bp* h, int dtype, const void* proj, int W, int H, long long view_stride,
+              const float* mats, const float* scale_offset, int nproj) {
+  FDK_REQUIRE(nproj >= 0 && nproj <= kMaxProj, "nProj %d exceeds %d views per pass", nproj, kMaxProj);
+  FDK_REQUIRE(mats != nullptr || nproj == 0, "matrices are NULL");
+  if (nproj == 0) return FDK_OK;
+  FDK_REQUIRE(W >= 2 && H >= 2, "views must be at least 2 x 2");
+  BpMats M;
+  BpScale S;
+  std::memset(&M, 0, sizeof(M));
+  std::memset(&S, 0, sizeof(S));
+  bool general = false;
+  for (int s = 0; s < nproj; ++s) {
+    for (int e = 0; e < 12; ++e) M.m[s][e] = mats[12 * s + e];
+    if (M.m[s][2] != 0.0f || M.m[s][10] != 0.0f) general = true;
+    S.so[s][0] = scale_offset ? scale_offset[2 * s] : 1.0f;
+    S.so[s][1] = scale_offset ? scale_offset[2 * s + 1] : 0.0f;
+  }
+  BpArgs a;
+  fill_args(h, W, H, view_stride, nproj, proj, a);
+  h->count += nproj;
+  switch (dtype) {
+    case FDK_F32: return launch_bp_dt<FDK_F32>(h, a, M, S, general);
+    case FDK_F16: return launch_bp_dt<FDK_F16>(h, a, M, S, general);
+    case FDK_BF16: return launch_bp_dt<FDK_BF16>(h, a, M, S, general);
+    default: return fdk::fail(FDK_ERR_INVALID, "unknown dtype %d", dtype);
+  }
+}
+
+int check_box(const int64_t* b, int nx, int ny, int nz, const char* name) {
+  FDK_REQUIRE(b != nullptr, "%s box is NULL", name);
+  FDK_REQUIRE(b[0] >= 0 && b[1] >= 0 && b[2] >= 0, "%s box origin is negative", name);
+  FDK_REQUIRE(b[3] >= 0 && b[4] >= 0 && b[5] >= 0, "%s box size is negative", name);
+  FDK_REQUIRE(b[0] + b[3] <= nx && b[1] + b[4] <= ny && b[2] + b[5] <= nz,
+              "%s box exceeds the %d x %d x %d volume", name, nx, ny, nz);
+  return FDK_OK;
+}
+
+}  // namespace
+
+extern "C" int fdk_bp_create(int proj_width, int proj_height, int proj_count, int nx, int ny,
+                             int nz, const int64_t box_top[6], const int64_t box_bottom[6],
+                             fdk_bp_t* out) {
+  FDK_REQUIRE(out != nullptr, "out is NULL");
+  *out = nullptr;
+  FDK_REQUIRE(proj_width >= 2 && proj_height >= 2 && proj_count >= 1, "bad projection size");
+  FDK_REQUIRE(nx > 0 && ny > 0 && nz > 0, "bad volume size");
+  if (int r = check_box(box_top, nx, ny, nz, "top")) return r;
+  const int64_t zero_box[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t* bb = box_bottom ? box_bottom : zero_box;
+  if (bb[5] > 0) {
+    if (int r = check_box(bb, nx, ny, nz, "bottom")) return r;
+    // backprojectionFDK.cu:173-175: the bottom slab mirrors the top one.
+    FDK_REQUIRE(box_top[2] + bb[2] + bb[5] - 1 == nz - 1,
+                "bottom slab must mirror the top slab: top.z + bottom.maxZ == nz - 1");
+    FDK_REQUIRE(bb[5] == box_top[5] && bb[0] == box_top[0] && bb[1] == box_top[1] &&
+                    bb[3] == box_top[3] && bb[4] == box_top[4],
+                "bottom slab must have the top slab's x/y extent and depth");
+  }
+  FDK_REQUIRE(box_top[3] > 0 && box_top[4] > 0 && box_top[5] > 0, "top box is empty");
+  fdk_bp* h = new fdk_bp();
+  h->proj_w = proj_width;
+  h->proj_h = proj_height;
+  h->proj_count = proj_count;
+  h->nx = nx; h->ny = ny; h->nz = nz;
+  h->top = Box{box_top[0], box_top[1], box_top[2], box_top[3], box_top[4], box_top[5]};
+  h->bot = Box{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5]};
+  h->sym = bb[5] > 0;
+  h->nzz = (int)(h->top.d + (h->sym ? h->bot.d : 0));
+  (void)hipGetDevice(&h->device);
+  const size_t nvox = (size_t)h->top.w * h->top.h * h->nzz;
+  hipError_t e = hipMalloc(&h->vol, nvox * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(h->vol, 0, nvox * sizeof(float));
+  h->staging_elems = (size_t)proj_width * proj_height * proj_count;
+  if (e == hipSuccess) e = hipMalloc(&h->staging, h->staging_elems * sizeof(float));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) h->own_stream = true;
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    fdk_bp_destroy(h);
+    return fdk::fail(e == hipErrorOutOfMemory ? FDK_ERR_OOM : FDK_ERR_HIP,
+                     "fdk_bp_create: %s (volume %zu voxels)", hipGetErrorString(e), nvox);
+  }
+  *out = h;
+  return FDK_OK;
+}
+
+extern "C" int fdk_bp_destroy(fdk_bp_t h) {
+  if (!h) return FDK_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->vol) (void)hipFree(h->vol);
+  if (h->staging) (void)hipFree(h->staging);
+  if (h->staging_t) (void)hipFree(h->staging_t);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FDK_OK;
+}
+
+extern "C" int fdk_bp_set_stream(fdk_bp_t h, void* stream) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_HIP(hipStreamSynchronize(h->stream));
+  if (h->own_stream) (void)hipStreamDestroy(h->stream);
+  h->stream = static_cast<hipStream_t>(stream);
+  h->own_stream = false;
+  return FDK_OK;
+}
+
+extern "C" int fdk_bp_batch(fdk_bp_t h, int transpose, const float* host_proj, int width,
+                            int height, const float* mats, int nproj) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_REQUIRE(transpose == 0 || transpose == 1, "transpose must be 0 or 1");
+  FDK_REQUIRE(host_proj != nullptr || nproj == 0, "projections are NULL");
+  FDK_REQUIRE(nproj <= kMaxProj && nproj <= h->proj_count, "nProj %d exceeds the %d-view batch",
+              nproj, std::min(kMaxProj, h->proj_count));
+  if (nproj <= 0) return FDK_OK;
+  // Stored layout the kernel reads: W = nv columns (v fastest) by H = nu rows.
+  const int W = transpose ? width : height;
+  const int H = transpose ? height : width;
+  // The constructor's projWidth x projHeight x projCount sizes the staging
+  // buffer (backprojectionFDK.cu:181); a call may use any view that fits it.
+  FDK_REQUIRE((size_t)width * height <= (size_t)h->proj_w * h->proj_h,
+              "view %d x %d exceeds the handle's %d x %d", width, height, h->proj_w, h->proj_h);
+  const size_t view = (size_t)width * height;
+  if (transpose) {
+    FDK_HIP(hipMemcpyAsync(h->staging, host_proj, view * nproj * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+  } else {
+    if (!h->staging_t) FDK_HIP(hipMalloc(&h->staging_t, h->staging_elems * sizeof(float)));
+    FDK_HIP(hipMemcpyAsync(h->staging_t, host_proj, view * nproj * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+    if (int r = launch_transpose(h->staging_t, h->staging, height, width, nproj, view, view, h->stream))
+      return r;
+  }
+  FDK_HIP(hipEventRecord(h->ev0, h->stream));
+  if (int r = launch_bp(h, FDK_F32, h->staging, W, H, (long long)view, mats, nullptr, nproj)) return r;
+  FDK_HIP(hipEventRecord(h->ev1, h->stream));
+  FDK_HIP(hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  FDK_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->kernel_ms += ms;
+  return FDK_OK;
+}
+
+extern "C" int fdk_bp_batch_device(fdk_bp_t h, int transpose, const void* dev_proj, int dtype,
+                                   int width, int height, int64_t view_stride, const float* mats,
+                                   const float* scale_offset, int nproj) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_REQUIRE(dev_proj != nullptr || nproj == 0, "projections are NULL");
+  FDK_REQUIRE(transpose == 0 || transpose == 1, "transpose must be 0 or 1");
+  FDK_REQUIRE(dtype == FDK_F32 || transpose == 1, "FP16/BF16 views must be stored transposed");
+  if (nproj <= 0) return FDK_OK;
+  FDK_REQUIRE(nproj <= kMaxProj, "nProj %d exceeds %d views per pass", nproj, kMaxProj);
+  const int W = transpose ? width : height;
+  const int H = transpose ? height : width;
+  FDK_REQUIRE(view_stride >= (int64_t)width * height, "view_stride %lld is smaller than a view",
+              (long long)view_stride);
+  if (transpose)
+    return launch_bp(h, dtype, dev_proj, W, H, (long long)view_stride, mats, scale_offset, nproj);
+  FDK_REQUIRE(nproj <= h->proj_count, "nProj exceeds the staging batch");
+  const size_t view = (size_t)width * height;
+  if (int r = launch_transpose(static_cast<const float*>(dev_proj), h->staging, height, width,
+                               nproj, view_stride, view, h->stream))
+    return r;
+  return launch_bp(h, FDK_F32, h->staging, W, H, (long long)view, mats, nullptr, nproj);
+}
+
+extern "C" int fdk_bp_sync(fdk_bp_t h) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_HIP(hipStreamSynchronize(h->stream));
+  return FDK_OK;
+}
+
+extern "C" int fdk_bp_count(fdk_bp_t h) { return h ? h->count : fdk::fail(FDK_ERR_INVALID, "NULL"); }
+
+extern "C" float fdk_bp_kernel_ms(fdk_bp_t h) { return h ? h->kernel_ms : 0.f; }
+
+extern "C" int fdk_bp_zero(fdk_bp_t h) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_HIP(hipMemsetAsync(h->vol, 0, (size_t)h->top.w * h->top.h * h->nzz * sizeof(float), h->stream));
+  h->count = 0;
+  h->kernel_ms = 0.f;
+  return FDK_OK;
+}
+
+namespace {
+
+int volume_out(fdk_bp* h, float* dst, size_t count, bool device_dst) {
+  const size_t plane = (size_t)h->top.w * h->top.h;
+  FDK_REQUIRE(dst != nullptr, "destination is NULL");
+  FDK_REQUIRE(count >= plane * h->nzz, "destination holds %zu floats, need %zu", count,
+              plane * h->nzz);
+  const int nxs = (int)h->top.w, nys = (int)h->top.h;
+  if (device_dst) {
+    dim3 grid((nxs + 31) / 32, (h->nzz + 31) / 32, nys);
+    hipLaunchKernelGGL(volume_out_kernel, grid, dim3(32, 8), 0, h->stream, h->vol, dst, nxs, nys,
+                       h->nzz, 0, h->nzz);
+    FDK_HIP(hipGetLastError());
+    FDK_HIP(hipStreamSynchronize(h->stream));
+    return FDK_OK;
+  }
+  // Host: transpose through a bounded device buffer, chunk by chunk of z planes
+  // (the reference halves its staging depth until it fits, backprojectionFDK.cu:206-212).
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>(h->nzz, (256u << 20) / (plane * 4)));
+  float* tmp = nullptr;
+  FDK_HIP(hipMalloc(&tmp, plane * chunk * sizeof(float)));
+  int rc = FDK_OK;
+  for (int z = 0; z < h->nzz && rc == FDK_OK; z += chunk) {
+    const int nzc = std::min(chunk, h->nzz - z);
+    dim3 grid((nxs + 31) / 32, (nzc + 31) / 32, nys);
+    hipLaunchKernelGGL(volume_out_kernel, grid, dim3(32, 8), 0, h->stream, h->vol, tmp, nxs, nys,
+                       h->nzz, z, nzc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dst + plane * z, tmp, plane * nzc * sizeof(float), hipMemcpyDeviceToHost,
+                         h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fdk::fail(FDK_ERR_HIP, "GetVolumeData: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int fdk_bp_get_volume(fdk_bp_t h, float* host_dst, size_t count) {
+  FDK_REQUIRE(h, "handle is NULL");
+  return volume_out(h, host_dst, count, false);
+}
+
+extern "C" int fdk_bp_get_volume_device(fdk_bp_t h, float* dev_dst, size_t count) {
+  FDK_REQUIRE(h, "handle is NULL");
+  return volume_out(h, dev_dst, count, true);
+}
+
+extern "C" int fdk_bp_volume_ptr(fdk_bp_t h, void** dev_ptr, int64_t dims[3]) {
+  FDK_REQUIRE(h && dev_ptr && dims, "NULL argument");
+  *dev_ptr = h->vol;
+  dims[0] = h->top.w;
+  dims[1] = h->top.h;
+  dims[2] = h->nzz;
+  return FDK_OK;
+}
+
+// ------------------------------------------------------------------ RTK path
+namespace {
+
+struct RtkArgs {
+  const float* vin;
+  float* vout;
+  const float* proj;  // [s][v][u], u fastest (W = nu, H = nv)
+  int W, H, nproj;
+  int nx, ny, nz;
+};
+
+// kernel_fdk_3Dgrid: one thread per voxel, x fastest across lanes, full
+// projective map per voxel, vol_out = vol_in + sum (rtk...cu:156-191).
+__global__ __launch_bounds__(256) void rtk_kernel(RtkArgs a, BpMats M) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= a.nx || j >= a.ny || k >= a.nz) return;
+  const float x = (float)i, y = (float)j, z = (float)k;
+  float acc = 0.f;
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    const float Z = 1.0f / fmaf(m[8], x, fmaf(m[9], y, fmaf(m[10], z, m[11])));
+    const float U = fmaf(m[0], x, fmaf(m[1], y, fmaf(m[2], z, m[3]))) * Z;
+    const float V = fmaf(m[4], x, fmaf(m[5], y, fmaf(m[6], z, m[7]))) * Z;
+    if (U >= 0.f && U < (float)(a.W - 1) && V >= 0.f && V < (float)(a.H - 1)) {
+      const float fu0 = floorf(U), fv0 = floorf(V);
+      const float fu = U - fu0, fv = V - fv0;
+      const float* p = a.proj + (long long)s * a.W * a.H + (long long)(int)fv0 * a.W + (int)fu0;
+      const float r0 = fmaf(fu, p[1] - p[0], p[0]);
+      const float r1 = fmaf(fu, p[a.W + 1] - p[a.W], p[a.W]);
+      acc = fmaf(fmaf(fv, r1 - r0, r0), Z * Z, acc);
+    }
+  }
+  const long long idx = ((long long)k * a.ny + j) * a.nx + i;
+  a.vout[idx] = a.vin[idx] + acc;
+}
+
+int launch_rtk(const int proj_size[3], const int vol_size[3], const float* mats,
+               const float* vin, float* vout, const float* dproj, hipStream_t st) {
+  FDK_REQUIRE(proj_size[2] >= 0 && proj_size[2] <= kMaxProj, "nProj %d exceeds %d", proj_size[2], kMaxProj);
+  FDK_REQUIRE(proj_size[0] >= 2 && proj_size[1] >= 2, "views must be at least 2 x 2");
+  FDK_REQUIRE(vol_size[0] > 0 && vol_size[1] > 0 && vol_size[2] > 0, "bad volume size");
+  BpMats M;
+  std::memset(&M, 0, sizeof(M));
+  for (int s = 0; s < proj_size[2]; ++s)
+    for (int e = 0; e < 12; ++e) M.m[s][e] = mats[12 * s + e];
+  RtkArgs a{vin, vout, dproj, proj_size[0], proj_size[1], proj_size[2], vol_size[0], vol_size[1], vol_size[2]};
+  dim3 grid((vol_size[0] + 63) / 64, (vol_size[1] + 3) / 4, vol_size[2]);
+  hipLaunchKernelGGL(rtk_kernel, grid, dim3(64, 4), 0, st, a, M);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+}  // namespace
+
+struct fdk_rtk {
+  int proj_w, proj_h, proj_count, nx, ny, nz;
+  float* vin = nullptr;
+  float* vout = nullptr;
+  float* staging = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float kernel_ms = 0.f;
+};
+
+extern "C" int fdk_rtk_create(int proj_width, int proj_height, int proj_count, int nx, int ny,
+                              int nz, fdk_rtk_t* out) {
+  FDK_REQUIRE(out, "out is NULL");
+  *out = nullptr;
+  FDK_REQUIRE(proj_width >= 2 && proj_height >= 2 && proj_count >= 1, "bad projection size");
+  FDK_REQUIRE(nx > 0 && ny > 0 && nz > 0, "bad volume size");
+  fdk_rtk* h = new fdk_rtk{proj_width, proj_height, proj_count, nx, ny, nz};
+  const size_t nvox = (size_t)nx * ny * nz;
+  hipError_t e = hipMalloc(&h->vin, nvox * 4);
+  if (e == hipSuccess) e = hipMalloc(&h->vout, nvox * 4);
+  if (e == hipSuccess) e = hipMemset(h->vin, 0, nvox * 4);
+  if (e == hipSuccess) e = hipMemset(h->vout, 0, nvox * 4);
+  if (e == hipSuccess) e = hipMalloc(&h->staging, (size_t)proj_width * proj_height * proj_count * 4);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    fdk_rtk_destroy(h);
+    return fdk::fail(e == hipErrorOutOfMemory ? FDK_ERR_OOM : FDK_ERR_HIP, "fdk_rtk_create: %s",
+                     hipGetErrorString(e));
+  }
+  *out = h;
+  return FDK_OK;
+}
+
+extern "C" int fdk_rtk_destroy(fdk_rtk_t h) {
+  if (!h) return FDK_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->vin) (void)hipFree(h->vin);
+  if (h->vout) (void)hipFree(h->vout);
+  if (h->staging) (void)hipFree(h->staging);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FDK_OK;
+}
+
+extern "C" int fdk_rtk_bp(fdk_rtk_t h, int transpose, const float* host_proj, int width,
+                          int height, const float* mats, int nproj) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_REQUIRE(transpose == 0, "BackProjectionRTK takes untransposed views (TRANSPOSE == 0)");
+  FDK_REQUIRE(width == h->proj_w && height == h->proj_h, "view size does not match the handle");
+  FDK_REQUIRE(nproj >= 0 && nproj <= std::min(kMaxProj, h->proj_count), "bad nProj %d", nproj);
+  if (nproj == 0) return FDK_OK;
+  FDK_HIP(hipMemcpyAsync(h->staging, host_proj, (size_t)width * height * nproj * 4,
+                         hipMemcpyHostToDevice, h->stream));
+  std::swap(h->vin, h->vout);  // SwapInOutBuffer (rtk...cu:391)
+  const int ps[3] = {width, height, nproj};
+  const int vs[3] = {h->nx, h->ny, h->nz};
+  FDK_HIP(hipEventRecord(h->ev0, h->stream));
+  if (int r = launch_rtk(ps, vs, mats, h->vin, h->vout, h->staging, h->stream)) return r;
+  FDK_HIP(hipEventRecord(h->ev1, h->stream));
+  FDK_HIP(hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  FDK_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->kernel_ms += ms;
+  return FDK_OK;
+}
+
+extern "C" int fdk_rtk_get_volume(fdk_rtk_t h, float* host_dst, size_t count) {
+  FDK_REQUIRE(h && host_dst, "NULL argument");
+  const size_t nvox = (size_t)h->nx * h->ny * h->nz;
+  FDK_REQUIRE(count >= nvox, "destination holds %zu floats, need %zu", count, nvox);
+  FDK_HIP(hipMemcpyAsync(host_dst, h->vout, nvox * 4, hipMemcpyDeviceToHost, h->stream));
+  FDK_HIP(hipStreamSynchronize(h->stream));
+  return FDK_OK;
+}
+
+extern "C" float fdk_rtk_kernel_ms(fdk_rtk_t h) { return h ? h->kernel_ms : 0.f; }
+
+extern "C" int fdk_reconstruct_conebeam(const int proj_size[3], const int vol_size[3],
+                                        const float* mats, const float* dev_vol_in,
+                                        float* dev_vol_out, const float* proj, int proj_on_device,
+                                        void* stream) {
+  FDK_REQUIRE(proj_size && vol_size && mats && dev_vol_in && dev_vol_out && proj, "NULL argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (proj_on_device) return launch_rtk(proj_size, vol_size, mats, dev_vol_in, dev_vol_out, proj, st);
+  const size_t bytes = (size_t)proj_size[0] * proj_size[1] * std::max(proj_size[2], 0) * 4;
+  float* tmp = nullptr;
+  FDK_HIP(hipMalloc(&tmp, std::max<size_t>(bytes, 4)));
+  hipError_t e = hipMemcpyAsync(tmp, proj, bytes, hipMemcpyHostToDevice, st);
+  int r = e == hipSuccess ? launch_rtk(proj_size, vol_size, mats, dev_vol_in, dev_vol_out, tmp, st)
+                          : fdk::fail(FDK_ERR_HIP, "upload: %s", hipGetErrorString(e));
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  return r;
+}

@@ -1,0 +1,34 @@
+// Shared host-side plumbing for the fdk_amd library: error state, status
+// macros and small device-memory RAII.  No reference counterpart beyond the
+// reference's VERIFY_TRUE / CUDA_CHECK_ERROR (common/type.h:194-196,
+// common/cudaLib.cuh:49-56), which throw; this library returns status codes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "fdk_amd.h"
+
+namespace fdk {
+
+// Records a printf-style message as the calling thread's last error and
+// returns code, so call sites can `return fail(...)`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+}  // namespace fdk
+
+#define FDK_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t fdk_e_ = (call);                                                        \
+    if (fdk_e_ != hipSuccess)                                                          \
+      return ::fdk::fail(fdk_e_ == hipErrorOutOfMemory ? FDK_ERR_OOM : FDK_ERR_HIP,    \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(fdk_e_),    \
+                         __FILE__, __LINE__);                                          \
+  } while (0)
+
+#define FDK_REQUIRE(cond, ...)                                \
+  do {                                                        \
+    if (!(cond)) return ::fdk::fail(FDK_ERR_INVALID, __VA_ARGS__); \
+  } while (0)

@@ -1,0 +1,217 @@
+/* fdk_amd -- MI355X (gfx950) FDK cone-beam reconstruction: C ABI.
+ *
+ * Plain C types only: pointers, sizes, ints, doubles.  Every entry point
+ * returns an int status (FDK_OK == 0, negative on error) unless noted; the
+ * message of the last error on the calling thread is fdk_last_error().
+ * "dev" pointers are device (HBM) addresses, "host" pointers host memory.
+ * "stream" is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Reference interfaces replaced (peng-ac/fdk, file:line):
+ *   fdk_geom_*           Parameter::init / CalculateCosTable / filter() /
+ *                        CalculateProjMat        common/type.h:4719-4944, 4427-4475
+ *   fdk_bp_*             BackProjectionFDK ctor / BP / GetVolumeData
+ *                                                 cudaBP/backprojection.cuh:246-271,
+ *                                                 cudaBP/backprojectionFDK.cu:161-302
+ *   fdk_rtk_*            BackProjectionRTK ctor / BP / GetVolumeData
+ *                                                 cudaBP/backprojection.cuh:180-198,
+ *                                                 rtkCudaFDKBackProjectionImageFilter.cu:340-400
+ *   fdk_reconstruct_conebeam
+ *                        BackProjectionRTK::CUDA_reconstruct_conebeam
+ *                                                 rtkCudaFDKBackProjectionImageFilter.cu:200-330
+ *   fdk_correct_image    CorrectImage_Real32      FilterlibCufft/cudaFftFilter.cuh:9
+ *   fdk_filter_rows      FilterRow_Real32         FilterlibCufft/cudaFftFilter.cuh:7
+ *   fdk_filter_views     fused cos weight + row filter (+ transpose / FP16 / BF16 store),
+ *                        the device half of FilterProjImage (cudaFftFilter.cu:394-452)
+ *                        and FilterEngine::Filtering (Filterlib/FilterLib.h:167-200)
+ *   fdk_filter_proj_image FilterProjImage (host in, host out, in place)
+ *                                                 FilterlibCufft/cudaFftFilter.cu:394
+ *   fdk_fp_project       cudaCTForwardProjection_f32 / ForwardProjection
+ *                                                 genProj/cudaIterType.cuh:280,
+ *                                                 genProj/cudaForwardProjection.cu:338-487
+ */
+#ifndef FDK_AMD_H
+#define FDK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDK_OK 0
+#define FDK_ERR_INVALID (-1)
+#define FDK_ERR_OOM (-2)
+#define FDK_ERR_HIP (-3)
+#define FDK_ERR_UNSUPPORTED (-4)
+
+/* Stored sample types of projections. */
+#define FDK_F32 0
+#define FDK_F16 1
+#define FDK_BF16 2
+
+/* Row padding before the FFT: zero, mean of the 3 or 4 edge samples. */
+#define FDK_PAD_ZERO 0
+#define FDK_PAD_MEAN3 3  /* FilterlibCufft kernelPad, type.h filterImageRow */
+#define FDK_PAD_MEAN4 4  /* Filterlib FilterImageRow (the CPU reference) */
+
+/* Forward-projector sampling modes. */
+#define FDK_FP_CLAMP 0   /* genProj GPU: t < tfar, clamp-to-edge trilinear */
+#define FDK_FP_ZERO 1    /* 3dct CPU:   t <= tfar, zero outside the volume */
+
+/* Scan geometry, the reference's Parameter (common/type.h:4627-4895).
+ * NOTE the reference naming kept here: sod = source-to-object distance
+ * (reference SOD), sid = source-to-detector distance (reference SID), mm. */
+typedef struct fdk_geometry {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  int nu, nv;
+  double du, dv;
+  double offset_nu, offset_nv;
+  double sod, sid;
+  int direction;
+  double scan_angle_deg;
+  int nproj;
+  double volume_position_angle_rad;
+  int step;
+  int filter_type; /* 0 none, 1 ram-lak, 2 shepp-logan, 3 cosine, 4 hamming, 5 hann */
+} fdk_geometry;
+
+/* Random-phantom ellipsoid (no reference counterpart): additive density inside
+ * |R^T (p - c) / a| <= 1, p in the volume frame (mm). */
+typedef struct fdk_ellipsoid {
+  double cx, cy, cz;
+  double ax, ay, az;
+  double rot[9];
+  double density;
+} fdk_ellipsoid;
+
+int fdk_last_error(char* buf, size_t len);
+int fdk_version(void);
+/* Number of HIP devices visible (0 when none); never fails. */
+int fdk_device_count(void);
+
+/* ------------------------------------------------------------ geometry (host, FP64) */
+/* type.h:4736 */
+int fdk_geom_angular_ramp_factor(const fdk_geometry* g, double* out);
+/* type.h:4756-4785: nv x nu floats, u fastest. */
+int fdk_geom_cos_table(const fdk_geometry* g, float* out);
+/* type.h:4418-4425: FFT length N = nextpow2(nu) (strictly greater power of two). */
+int fdk_geom_filter_length(const fdk_geometry* g);
+/* type.h:4427-4475: N doubles. */
+int fdk_geom_filter(const fdk_geometry* g, double* out, int cap);
+/* type.h:4732, 4824: view angle in rad (direction * deg + volume_position_angle). */
+int fdk_geom_view_angle(const fdk_geometry* g, int view, double* out);
+/* type.h:4898-4944: count matrices (3x4 row-major doubles) for views first.. */
+int fdk_geom_proj_matrices(const fdk_geometry* g, int first, int count, double* out);
+
+/* ------------------------------------------------------------ backprojection (FDK) */
+typedef struct fdk_bp* fdk_bp_t;
+#define FDK_BP_MAX_PROJ 32
+
+/* BackProjectionFDK(projWidth, projHeight, projCount, nx, ny, nz, subTop, subBottom).
+ * box_top / box_bottom: {x, y, z, width, height, depth}; box_bottom depth 0 means
+ * no mirror slab.  With a mirror slab box_top.z + box_bottom.z + box_bottom.depth
+ * must equal nz (backprojectionFDK.cu:173-175).  Allocates the volume
+ * (zeroed) and a projCount-view staging buffer on the current device. */
+int fdk_bp_create(int proj_width, int proj_height, int proj_count, int nx, int ny, int nz,
+                  const int64_t box_top[6], const int64_t box_bottom[6], fdk_bp_t* out);
+int fdk_bp_destroy(fdk_bp_t h);
+/* Stream for all later work of the handle (default: a stream the handle owns). */
+int fdk_bp_set_stream(fdk_bp_t h, void* stream);
+/* BackProjectionFDK::BP (backprojectionFDK.cu:229-276): synchronous.
+ * transpose 1: each view is width (= nv) columns x height (= nu) rows, v fastest;
+ * transpose 0: width (= nu) x height (= nv), u fastest.  mats: nproj x 12 floats
+ * acting on (x, y, z, 1) voxel indices.  nproj <= FDK_BP_MAX_PROJ. */
+int fdk_bp_batch(fdk_bp_t h, int transpose, const float* host_proj, int width, int height,
+                 const float* mats, int nproj);
+/* Device-resident batch, asynchronous on the handle's stream.  dev_proj holds
+ * nproj views view_stride elements apart, transposed layout (width = nv
+ * contiguous, height = nu rows) for FDK_F16/FDK_BF16; FDK_F32 also takes
+ * transpose 0.  scale_offset (host, 2*nproj floats or NULL) maps stored samples
+ * q to projection values p = q*scale + offset per view. */
+int fdk_bp_batch_device(fdk_bp_t h, int transpose, const void* dev_proj, int dtype, int width,
+                        int height, int64_t view_stride, const float* mats,
+                        const float* scale_offset, int nproj);
+int fdk_bp_sync(fdk_bp_t h);
+/* Views backprojected so far (nBpCount) and summed kernel time (fKernelTime, ms;
+ * only synchronous calls record it). */
+int fdk_bp_count(fdk_bp_t h);
+float fdk_bp_kernel_ms(fdk_bp_t h);
+int fdk_bp_zero(fdk_bp_t h);
+/* GetVolumeData: host [z over top then bottom][y][x], count floats. */
+int fdk_bp_get_volume(fdk_bp_t h, float* host_dst, size_t count);
+/* Same layout into device memory (dst must hold the whole sub-volume). */
+int fdk_bp_get_volume_device(fdk_bp_t h, float* dev_dst, size_t count);
+/* The working volume itself: device pointer and its extents {nxs, nys, nzz};
+ * layout [x][y][zz] with zz fastest (zz: the top slab, then the bottom slab,
+ * each in increasing z).  Valid until fdk_bp_destroy. */
+int fdk_bp_volume_ptr(fdk_bp_t h, void** dev_ptr, int64_t dims[3]);
+
+/* ------------------------------------------------------------ backprojection (RTK) */
+typedef struct fdk_rtk* fdk_rtk_t;
+/* BackProjectionRTK(projWidth, projHeight, projCount, nx, ny, nz): volume
+ * [z][y][x] x fastest, in/out ping-pong as the reference. */
+int fdk_rtk_create(int proj_width, int proj_height, int proj_count, int nx, int ny, int nz,
+                   fdk_rtk_t* out);
+int fdk_rtk_destroy(fdk_rtk_t h);
+/* BackProjectionRTK::BP: transpose must be 0; host projections width (= nu) x
+ * height (= nv); synchronous. */
+int fdk_rtk_bp(fdk_rtk_t h, int transpose, const float* host_proj, int width, int height,
+               const float* mats, int nproj);
+int fdk_rtk_get_volume(fdk_rtk_t h, float* host_dst, size_t count);
+float fdk_rtk_kernel_ms(fdk_rtk_t h);
+/* CUDA_reconstruct_conebeam: vol_out = vol_in + backprojection of proj_size[2]
+ * views (<= FDK_BP_MAX_PROJ) of proj_size[0] x proj_size[1] samples.  Volumes are
+ * device pointers; proj is host memory when proj_on_device is 0. */
+int fdk_reconstruct_conebeam(const int proj_size[3], const int vol_size[3], const float* mats,
+                             const float* dev_vol_in, float* dev_vol_out, const float* proj,
+                             int proj_on_device, void* stream);
+
+/* ------------------------------------------------------------ filtering */
+/* CorrectImage_Real32: dst = src * table over depth images (table broadcast);
+ * pitch, table_pitch in elements. dst may equal src. */
+int fdk_correct_image(const float* dev_src, const float* dev_table, float* dev_dst, int width,
+                      int height, int depth, int pitch, int table_pitch, void* stream);
+/* FilterRow_Real32: each of height*depth rows of width samples (pitch apart) is
+ * padded into N (power of two, 16..16384) per pad_mode, filtered by the real
+ * length-N filter H and cropped back; dst may equal src. */
+int fdk_filter_rows(const float* dev_src, float* dev_dst, int width, int height, int depth,
+                    int src_pitch, int dst_pitch, const float* dev_filter, int N, int pad_mode,
+                    void* stream);
+/* Fused cos weighting + row filter + optional transpose + stored dtype.
+ * src: nviews x nv x nu floats (u fastest, view_stride_in elements apart).
+ * dev_cos: nv x nu floats or NULL (no weighting).  dev_filter: N floats.
+ * transpose 1 writes each view as nu rows of nv (v fastest, the layout
+ * fdk_bp_batch_device reads); 0 keeps nv rows of nu.  dst may equal src only
+ * when transpose is 0 and dtype is FDK_F32.  dtype FDK_F16/FDK_BF16
+ * stores (p - min)/(max - min) per view and writes {max - min, min} per view to
+ * host_scale_offset (2*nviews floats); the call then synchronizes the stream. */
+int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst,
+                     int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
+                     const float* dev_cos, const float* dev_filter, int N, int pad_mode,
+                     int transpose, float* host_scale_offset, void* stream);
+/* FilterProjImage: host views (nviews x nv x nu) filtered in place, in batches of
+ * depth views (32 by default when depth <= 0), cos weighting by host_cos. */
+int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nviews, const float* host_cos,
+                          const float* host_filter, int N, int pad_mode, int depth);
+
+/* ------------------------------------------------------------ forward projection */
+/* Ray-marching projection of views [first, first+count) of geometry g through
+ * dev_vol (nz x ny x nx floats, x fastest; sizes and spacing from g), step_mm
+ * (<= 0: min(dx,dy,dz), the reference's step).  dev_out: count x nv x nu floats. */
+int fdk_fp_project(const fdk_geometry* g, const float* dev_vol, int first, int count,
+                   double step_mm, int mode, float* dev_out, void* stream);
+
+/* ------------------------------------------------------------ random phantoms */
+/* Voxel-centre rasterization into dev_vol (nz x ny x nx floats, overwritten). */
+int fdk_phantom_rasterize(const fdk_ellipsoid* host_ell, int n, const fdk_geometry* g,
+                          float* dev_vol, void* stream);
+/* Exact line integrals for views [first, first+count): dev_out count x nv x nu. */
+int fdk_phantom_project(const fdk_ellipsoid* host_ell, int n, const fdk_geometry* g, int first,
+                        int count, float* dev_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDK_AMD_H */
