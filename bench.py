@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""FDK backprojection throughput on one MI355X: 2048^3 FP32 volume from
+2048x2048-pixel views, 4096 views over 360 degrees (SOD 1000 mm, SDD 1500 mm,
+0.2 mm pixels, 0.1333 mm voxels), projections and volume resident in HBM.
+
+One step = one 32-view backprojection pass over the whole volume (the
+reference's BackProjectionFDK::BP batch).  Projections are exact line integrals
+of a seeded 40-ellipsoid phantom, cos-weighted and ramp/Hann filtered on the
+GPU before the timed region; the timed region holds only backprojection passes.
+
+Prints one JSON line (see DESIGN.md for every field's definition).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "backprojection GUPS at 2048^3/4096 views; learned-vs-plain recon time @ RMSE"
+REFERENCE_BP_GUPS_V100 = 150.9  # BP GUPS per V100, same problem (reference .t2 run artefacts)
+HBM_PEAK = 8.0e12               # MI355X HBM3E, B/s
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2048, help="volume edge (and detector edge)")
+    ap.add_argument("--views", type=int, default=4096)
+    ap.add_argument("--dtype", choices=["f32", "f16", "bf16"], default="f32")
+    ap.add_argument("--resident-views", type=int, default=0,
+                    help="views generated and kept in HBM (0: all views)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def geometry(n: int, views: int):
+    import fdk_amd as F
+    det = 0.2 * 2048 / n          # detector pitch keeps a 409.6 mm detector
+    vox = det * 1000.0 / 1500.0   # voxel = pixel at isocentre (magnification 1.5)
+    return F.Parameter(nx=n, ny=n, nz=n, dx=vox, dy=vox, dz=vox, nu=n, nv=n, du=det, dv=det,
+                       sod=1000.0, sid=1500.0, nproj=views, direction=-1, scan_angle_deg=360.0,
+                       volume_position_angle_rad=0.0, filter_type=5)
+
+
+def cpu_baseline(p, proj_t_host_fn, seconds: float):
+    """Reference CPU backprojection (3dct BackProjection<float>, compiled from
+    the reference sources into oracle/_ref) timed on this host's cores on the
+    central 256^3 sub-volume of the same geometry; falls back to the oracle
+    port when oracle/_ref is absent."""
+    import numpy as np
+    import oracle as O
+    sub = min(256, p.nx)
+    g = O.Geom(nx=sub, ny=sub, nz=sub, dx=p.dx, dy=p.dy, dz=p.dz, nu=p.nu, nv=p.nv, du=p.du,
+               dv=p.dv, sod=p.sod, sid=p.sid, direction=p.direction, nproj=p.nproj,
+               scan_angle_deg=p.scan_angle_deg, volume_position_angle_rad=0.0, filter_type=5)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    use_ref = O.ref_available()
+    probe = 2
+    frames = proj_t_host_fn(probe)
+    mats = O.proj_matrices(g)
+    t0 = time.perf_counter()
+    if use_ref:
+        O.ref_bp_3dct_views(g, frames, probe)
+    else:
+        O.bp(g, frames, mats[:probe], O.BP_GPU_STRICT)
+    per_view = (time.perf_counter() - t0) / probe
+    # bounded sample: about `seconds` of CPU work, at most 128 views (2 GiB of views)
+    nv = int(max(probe, min(128, p.nproj, seconds / max(per_view, 1e-6))))
+    frames = proj_t_host_fn(nv)
+    t0 = time.perf_counter()
+    if use_ref:
+        O.ref_bp_3dct_views(g, frames, nv)
+    else:
+        O.bp(g, frames, mats[:nv], O.BP_GPU_STRICT)
+    dt = time.perf_counter() - t0
+    gups = sub ** 3 * nv / dt / 1e9
+    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": (f"central {sub}^3 sub-volume of the {p.nx}^3 geometry x {nv} views "
+                       f"({sub ** 3 * nv:.3g} voxel-updates, {dt:.1f} s), prefiltered FP32 "
+                       "views, " + ("3dct BackProjection<float> compiled from the reference "
+                                    "sources" if use_ref else "oracle port (FP64)"))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here "
+              "has one MI355X; the multi-GPU sharding design is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    sys.path.insert(0, HERE)
+    import fdk_amd as F
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    n, V = args.n, args.views
+    p = geometry(n, V)
+    B = F.MAX_PROJ
+    dtype = {"f32": F.FDK_F32, "f16": F.FDK_F16, "bf16": F.FDK_BF16}[args.dtype]
+    need = (args.warmup + args.steps) * B
+    resident = V if args.resident_views <= 0 else min(V, max(args.resident_views, B))
+    resident = min(resident, V)
+    log(f"geometry {n}^3 voxels of {p.dx:.4f} mm, {n}^2 detector of {p.du:.3f} mm, {V} views; "
+        f"{resident} views resident ({args.dtype})")
+
+    # ---- inputs: exact projections of a seeded phantom, filtered on the GPU
+    ell = F.random_ellipsoids(40, seed=2, radius_mm=0.45 * n * p.dx, half_height_mm=0.45 * n * p.dz,
+                              axes_mm=(5.0 * p.dx / 0.1333, 100.0 * p.dx / 0.1333))
+    cos = torch.as_tensor(p.cos_table()).to(dev)
+    H = torch.as_tensor(p.filter().astype(np.float32)).to(dev)
+    tdt = torch.float32 if dtype == F.FDK_F32 else torch.int16
+    store = torch.empty((resident, p.nu, p.nv), dtype=tdt, device=dev)
+    raw = torch.empty((B, p.nv, p.nu), dtype=torch.float32, device=dev)
+    scale_offset = np.zeros((resident, 2), np.float32)
+    t0 = time.perf_counter()
+    for b in range(0, resident, B):
+        cnt = min(B, resident - b)
+        F.phantom_project(ell, p, b, cnt, raw)
+        so = F.filter_views(raw, store, cos, H, F.PAD_MEAN4, True, dtype=dtype, nviews=cnt,
+                            dst_view_offset=b)
+        if so is not None:
+            scale_offset[b:b + cnt] = so
+    torch.cuda.synchronize()
+    log(f"generated and filtered {resident} views in {time.perf_counter() - t0:.1f} s")
+    mats = p.proj_matrices().astype(np.float32)
+
+    # ---- the backprojector: mirror slab pair covering the whole volume
+    half = n // 2
+    bp = F.BackProjectionFDK(p.nv, p.nu, B, n, n, n, (0, 0, 0, n, n, half),
+                             (0, 0, half, n, n, half))
+    stream = torch.cuda.current_stream()
+    bp.set_stream(stream.cuda_stream)
+
+    def step(i):
+        v0 = (i * B) % V
+        s0 = v0 % resident
+        cnt = min(B, resident - s0)
+        bp.bp_device(store, p.nv, p.nu, mats[v0:v0 + cnt], cnt, dtype=dtype, offset_views=s0,
+                     scale_offset=None if dtype == F.FDK_F32 else scale_offset[s0:s0 + cnt])
+        return cnt
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    views_done = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        evs[i][0].record(stream)
+        views_done += step(args.warmup + i)
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in evs]
+    log(f"timed {args.steps} steps in {wall:.2f} s")
+
+    vol = bp.volume_tensor()
+    probe = vol[n // 2, n // 2, ::97].float().cpu().numpy()
+    if not np.all(np.isfinite(probe)):
+        log("non-finite voxels in the result")
+        sys.exit(1)
+
+    upd = float(n) ** 3 * views_done
+    gups = upd / wall / 1e9
+    avg_ms = float(np.mean(kern_ms))
+    s_bytes = 4 if dtype == F.FDK_F32 else 2
+    b_upd = 4 * s_bytes + 8.0 / B          # 4 taps + volume read/write per 32-view pass
+    algo_bytes = b_upd * float(n) ** 3 * B  # per launch (one 32-view pass)
+    achieved = algo_bytes / (avg_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", f"bp_{args.dtype}_{n}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sub = min(256, n)
+
+        def frames(k):
+            # the same filtered views, untransposed, for the CPU sub-volume
+            nk = min(k, resident)
+            x = store[:nk]
+            if dtype != F.FDK_F32:
+                return None
+            return np.ascontiguousarray(x.transpose(1, 2).cpu().numpy())
+        log("timing the reference CPU backprojection")
+        try:
+            cpu = cpu_baseline(p, frames, args.cpu_seconds) if dtype == F.FDK_F32 else None
+        except Exception as e:  # the baseline never decides the GPU result
+            log(f"cpu baseline failed: {e}")
+            cpu = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(gups, 2),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(gups / REFERENCE_BP_GUPS_V100, 3),
+        "dtype": {"f32": "f32", "f16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.dtype],
+        "data": "synthetic (exact line integrals of a seeded 40-ellipsoid phantom, filtered on the GPU)",
+        "config": {"workload": f"FDK backprojection, {n}^3 FP32 volume from {V} views of "
+                               f"{n}x{n} pixels, 32-view passes, all inputs resident in HBM",
+                   "volume": [n, n, n], "detector": [n, n], "views": V, "views_per_step": B,
+                   "voxel_mm": round(p.dx, 5), "pixel_mm": p.du, "sod_mm": p.sod, "sdd_mm": p.sid,
+                   "projection_storage": args.dtype, "resident_views": resident},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": traffic,
+                     "bytes_per_update": b_upd, "kernel_ms_avg": round(avg_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    bp.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,7 @@ def ref():
         _ref.ref_filter_design.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
         _ref.ref_fp_3dct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _ref.ref_bp_3dct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _ref.ref_bp_3dct_views.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         _ref.ref_filter_rows_3dct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                               C.c_int]
         _ref.ref_fft.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
@@ -307,6 +308,15 @@ def ref_bp_3dct(g: Geom, frames: np.ndarray, prefiltered: bool) -> np.ndarray:
     vol = np.zeros((g.nz, g.ny, g.nx), np.float32)
     gc = g.c()
     ref().ref_bp_3dct(C.byref(gc), _p(frames), 1 if prefiltered else 0, _p(vol))
+    return vol
+
+
+def ref_bp_3dct_views(g: Geom, frames: np.ndarray, nviews: int) -> np.ndarray:
+    """Reference 3dct BackProjection of prefiltered views [0, nviews) of g."""
+    frames = np.ascontiguousarray(frames, np.float32)
+    vol = np.zeros((g.nz, g.ny, g.nx), np.float32)
+    gc = g.c()
+    ref().ref_bp_3dct_views(C.byref(gc), _p(frames), 1, int(nviews), _p(vol))
     return vol
 
 

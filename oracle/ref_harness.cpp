@@ -81,7 +81,17 @@ int ref_fft(int dir, int m, double* re, double* im) {
 // frames: nproj x nv x nu floats (u fastest).  vol: nz x ny x nx (x fastest),
 // accumulated into.  prefiltered != 0 feeds already-filtered projections through
 // the str_filter=="none" branch, undoing its 1/(nProj/step) scale first.
+int ref_bp_3dct_views(const RefGeom* g, const float* frames, int prefiltered, int nviews,
+                      float* vol_out);
+
 int ref_bp_3dct(const RefGeom* g, const float* frames, int prefiltered, float* vol_out) {
+  return ref_bp_3dct_views(g, frames, prefiltered, g->nproj, vol_out);
+}
+
+// As ref_bp_3dct over views [0, nviews) only (the geometry keeps its nproj, so
+// the angles and weights are those of the full scan).
+int ref_bp_3dct_views(const RefGeom* g, const float* frames, int prefiltered, int nviews,
+                      float* vol_out) {
   Parameter p;
   RefGeom gg = *g;
   if (prefiltered) gg.filter_type = 0;
@@ -92,7 +102,7 @@ int ref_bp_3dct(const RefGeom* g, const float* frames, int prefiltered, float* v
   const size_t frame_size = (size_t)g->nu * g->nv;
   std::vector<float> buf(frame_size);
   const float undo = float(p.nProj) / p.step;
-  for (int s = 0; s < g->nproj; s += g->step) {
+  for (int s = 0; s < nviews && s < g->nproj; s += g->step) {
     for (size_t i = 0; i < frame_size; ++i)
       buf[i] = prefiltered ? frames[s * frame_size + i] * undo : frames[s * frame_size + i];
     FrameT<float> frame(buf.data(), g->nu, g->nv, s);
