@@ -99,6 +99,10 @@ _SIGS = {
                             C.c_void_p]),
     "fdk_filter_proj_image": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fdk_filter_frame_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fdk_fp_project_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                      C.c_int, C.c_void_p]),
     "fdk_fp_project": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_void_p, C.c_void_p]),
     "fdk_phantom_rasterize": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,

@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -64,6 +65,10 @@ struct BpArgs {
   int nzz;               // total zz extent of the buffer
   int nz;                // global nz (mirror voxel z = nz - 1 - z)
   int tiles_x;           // tiles along x
+  long long view_bytes;  // view_stride in bytes
+  int view_bytes32;      // W * H * sample bytes (< 2^31), the range of a view's resource
+  int row_bytes;         // W * sample bytes
+  int buf16;             // 16-bit samples with 4-byte aligned rows and views (buffer path)
 };
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
@@ -117,6 +122,65 @@ __device__ inline float sample_v(const void* rows, float v, int W, float w0, flo
   const float rb = fmaf(t, b1 - b0, b0);
   const float s = fmaf(w0, ra, w1 * rb);
   return ok ? s : 0.0f;
+}
+
+// Buffer-resource word 3 for untyped (raw) dword access on gfx9 (CDNA): data
+// format 32, no swizzle.
+constexpr int kRsrcWord3 = 0x00020000;
+
+// FP32 sample through the row-u / row-u+1 resources: one 32-bit offset, two
+// 8-byte loads (taps v, v+1 of each row).  0 outside 0 <= v < W-1.
+__device__ inline float sample_buf(__amdgpu_buffer_rsrc_t r0, __amdgpu_buffer_rsrc_t r1, float v,
+                                   int rowbase, int W, float w0, float w1) {
+  const float fv = floorf(v);
+  const int iv = (int)fv;
+  const bool ok = (unsigned)iv <= (unsigned)(W - 2);
+  const float t = v - fv;
+  const int off = (int)(((unsigned)rowbase + (unsigned)iv) << 2);  // wraps harmlessly when !ok
+  const float2 a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r0, off, 0, 0));
+  const float2 b = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r1, off, 0, 0));
+  const float ra = fmaf(t, a.y - a.x, a.x);
+  const float rb = fmaf(t, b.y - b.x, b.x);
+  return ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
+}
+
+// 16-bit sample (FP16 or BF16 stored, FP32 math) through the same two
+// resources.  The taps e = rowbase + iv and e + 1 sit in the 8-byte window that
+// starts at the 4-byte aligned element e & ~1; alignbyte picks the 32 bits that
+// hold them.  With an even row width W the window of every valid sample lies
+// inside its resource (a window that crosses the end only drops taps that the
+// sample does not use).  Returns the interpolated stored value q (weighted by
+// w0, w1) and sets *ok; the caller applies the per-view affine scale.
+template <int DT>
+__device__ inline float sample_buf16(__amdgpu_buffer_rsrc_t r0, __amdgpu_buffer_rsrc_t r1, float v,
+                                     int rowbase, int W, float w0, float w1, bool* okp) {
+  const float fv = floorf(v);
+  const int iv = (int)fv;
+  const bool ok = (unsigned)iv <= (unsigned)(W - 2);
+  const float t = v - fv;
+  const unsigned e = (unsigned)rowbase + (unsigned)iv;
+  const int off = (int)((e & ~1u) << 1);
+  const unsigned sh = (e & 1u) << 1;
+  const auto a = __builtin_amdgcn_raw_buffer_load_b64(r0, off, 0, 0);
+  const auto b = __builtin_amdgcn_raw_buffer_load_b64(r1, off, 0, 0);
+  const unsigned wa = __builtin_amdgcn_alignbyte(a[1], a[0], sh);
+  const unsigned wb = __builtin_amdgcn_alignbyte(b[1], b[0], sh);
+  float a0, a1, b0, b1;
+  if constexpr (DT == FDK_F16) {
+    a0 = __half2float(__ushort_as_half((unsigned short)(wa & 0xffffu)));
+    a1 = __half2float(__ushort_as_half((unsigned short)(wa >> 16)));
+    b0 = __half2float(__ushort_as_half((unsigned short)(wb & 0xffffu)));
+    b1 = __half2float(__ushort_as_half((unsigned short)(wb >> 16)));
+  } else {
+    a0 = __uint_as_float(wa << 16);
+    a1 = __uint_as_float(wa & 0xffff0000u);
+    b0 = __uint_as_float(wb << 16);
+    b1 = __uint_as_float(wb & 0xffff0000u);
+  }
+  const float ra = fmaf(t, a1 - a0, a0);
+  const float rb = fmaf(t, b1 - b0, b0);
+  *okp = ok;
+  return ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
 }
 
 // Block id remap so that consecutive logical blocks land on one XCD: hardware
@@ -173,21 +237,54 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
       w1 *= S.so[s][0];
       off = S.so[s][1] * Z2;  // bilinear weights sum to 1 inside the view
     }
-    const long long rowoff = (long long)s * a.view_stride + (long long)(uok ? iu : 0) * a.W;
-    const void* rows;
-    if (DT == FDK_F32) rows = static_cast<const float*>(a.proj) + rowoff;
-    else rows = static_cast<const unsigned short*>(a.proj) + rowoff;
+    if constexpr (DT == FDK_F32) {
+      // Two buffer resources per view: rows u (from the view start) and u+1
+      // (from one row later), each bounded by the view's end, so every lane
+      // offset is range-checked by the hardware and an out-of-view offset
+      // reads 0 instead of touching memory.  Lane offsets are 32-bit.
+      const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
+      const __amdgpu_buffer_rsrc_t r0 =
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+      const int rowbase = uok ? iu * a.W : 0;
 #pragma unroll
-    for (int j = 0; j < ZPT; ++j) {
-      const float v = fmaf(zt[j], dV, Vb);
-      float c = sample_v<DT>(rows, v, a.W, w0, w1);
-      if (DT != FDK_F32) c += (uok && v >= 0.f && v < (float)(a.W - 1)) ? off : 0.f;
-      acc[j] += c;
-      if (SYM) {
-        const float vm = fmaf(zm[j], dV, Vb);
-        float cm = sample_v<DT>(rows, vm, a.W, w0, w1);
-        if (DT != FDK_F32) cm += (uok && vm >= 0.f && vm < (float)(a.W - 1)) ? off : 0.f;
-        accm[j] += cm;
+      for (int j = 0; j < ZPT; ++j) {
+        acc[j] += sample_buf(r0, r1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0, w1);
+        if (SYM) accm[j] += sample_buf(r0, r1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0, w1);
+      }
+    } else if (a.buf16) {
+      const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
+      const __amdgpu_buffer_rsrc_t r0 =
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+      const int rowbase = uok ? iu * a.W : 0;
+#pragma unroll
+      for (int j = 0; j < ZPT; ++j) {
+        bool ok;
+        const float c = sample_buf16<DT>(r0, r1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0, w1, &ok);
+        acc[j] += (uok && ok) ? c + off : 0.f;
+        if (SYM) {
+          const float cm = sample_buf16<DT>(r0, r1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0, w1, &ok);
+          accm[j] += (uok && ok) ? cm + off : 0.f;
+        }
+      }
+    } else {
+      const long long rowoff = (long long)s * a.view_stride + (long long)(uok ? iu : 0) * a.W;
+      const void* rows = static_cast<const unsigned short*>(a.proj) + rowoff;
+#pragma unroll
+      for (int j = 0; j < ZPT; ++j) {
+        const float v = fmaf(zt[j], dV, Vb);
+        float c = sample_v<DT>(rows, v, a.W, w0, w1);
+        c += (uok && v >= 0.f && v < (float)(a.W - 1)) ? off : 0.f;
+        acc[j] += c;
+        if (SYM) {
+          const float vm = fmaf(zm[j], dV, Vb);
+          float cm = sample_v<DT>(rows, vm, a.W, w0, w1);
+          cm += (uok && vm >= 0.f && vm < (float)(a.W - 1)) ? off : 0.f;
+          accm[j] += cm;
+        }
       }
     }
   }
@@ -321,8 +418,10 @@ struct fdk_bp {
 
 namespace {
 
-int fill_args(fdk_bp* h, int W, int H, long long view_stride, int nproj, const void* proj,
-              BpArgs& a) {
+int fill_args(fdk_bp* h, int W, int H, long long view_stride, int esize, int nproj,
+              const void* proj, BpArgs& a) {
+  const long long view_bytes = (long long)W * H * esize;
+  FDK_REQUIRE(view_bytes < (1LL << 31), "a view of %d x %d samples exceeds 2 GiB", W, H);
   a.vol = h->vol;
   a.proj = proj;
   a.view_stride = view_stride;
@@ -337,26 +436,46 @@ int fill_args(fdk_bp* h, int W, int H, long long view_stride, int nproj, const v
   a.zext = h->sym ? (int)h->top.d : h->nzz;
   a.nzz = h->nzz;
   a.nz = h->nz;
-  const int npx = (a.nxs + 1) / 2, npy = (a.nys + 1) / 2;
-  a.tiles_x = (npx + kTile - 1) / kTile;
-  (void)npy;
+  a.tiles_x = ((a.nxs + 1) / 2 + kTile - 1) / kTile;
+  a.view_bytes = view_stride * esize;
+  a.view_bytes32 = (int)view_bytes;
+  a.row_bytes = W * esize;
+  a.buf16 = esize == 2 && (W % 2 == 0) && (view_stride % 2 == 0) &&
+            (reinterpret_cast<uintptr_t>(proj) % 4 == 0);
   return FDK_OK;
+}
+
+// Voxels per lane along z (tuning knob FDK_BP_ZPT = 4, 8 or 16; default 8).
+int zpt_choice() {
+  static const int z = [] {
+    const char* e = std::getenv("FDK_BP_ZPT");
+    const int v = e ? std::atoi(e) : 8;
+    return (v == 4 || v == 8 || v == 16) ? v : 8;
+  }();
+  return z;
+}
+
+template <int DT, int ZPT>
+void launch_column(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
+  dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
+  if (h->sym) hipLaunchKernelGGL((bp_column_kernel<DT, true, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
+  else hipLaunchKernelGGL((bp_column_kernel<DT, false, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
 }
 
 template <int DT>
 int launch_bp_dt(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, bool general) {
-  const int npy = (a.nys + 1) / 2;
-  const int tiles_y = (npy + kTile - 1) / kTile;
+  const int tiles_y = ((a.nys + 1) / 2 + kTile - 1) / kTile;
   const unsigned nblk = (unsigned)(a.tiles_x * tiles_y * kTile * kTile);
-  constexpr int ZPT = 4;
   if (general) {
     dim3 grid(nblk, (a.zext + kWave - 1) / kWave);
     if (h->sym) hipLaunchKernelGGL((bp_general_kernel<DT, true>), grid, dim3(256), 0, h->stream, a, M, S);
     else hipLaunchKernelGGL((bp_general_kernel<DT, false>), grid, dim3(256), 0, h->stream, a, M, S);
   } else {
-    dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
-    if (h->sym) hipLaunchKernelGGL((bp_column_kernel<DT, true, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
-    else hipLaunchKernelGGL((bp_column_kernel<DT, false, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
+    switch (zpt_choice()) {
+      case 4: launch_column<DT, 4>(h, a, M, S, nblk); break;
+      case 16: launch_column<DT, 16>(h, a, M, S, nblk); break;
+      default: launch_column<DT, 8>(h, a, M, S, nblk); break;
+    }
   }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
@@ -380,7 +499,7 @@ int launch_bp(fdk_bp* h, int dtype, const void* proj, int W, int H, long long vi
     S.so[s][1] = scale_offset ? scale_offset[2 * s + 1] : 0.0f;
   }
   BpArgs a;
-  fill_args(h, W, H, view_stride, nproj, proj, a);
+  if (int r = fill_args(h, W, H, view_stride, dtype == FDK_F32 ? 4 : 2, nproj, proj, a)) return r;
   h->count += nproj;
   switch (dtype) {
     case FDK_F32: return launch_bp_dt<FDK_F32>(h, a, M, S, general);

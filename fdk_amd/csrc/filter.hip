@@ -447,3 +447,34 @@ extern "C" int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nview
   (void)hipStreamDestroy(st);
   return r;
 }
+
+extern "C" int fdk_filter_frame_host(float* host_src, float* host_dst, int nu, int nv,
+                                     const float* host_cos, const float* host_filter, int N,
+                                     int pad_mode, int transpose) {
+  FDK_REQUIRE(host_src && host_dst && host_cos && host_filter, "NULL pointer");
+  FDK_REQUIRE(nu > 0 && nv > 0, "bad image size");
+  const size_t view = (size_t)nu * nv;
+  float *s = nullptr, *d = nullptr, *c = nullptr, *h = nullptr;
+  hipError_t e = hipMalloc(&s, view * 4);
+  if (e == hipSuccess) e = hipMalloc(&d, view * 4);
+  if (e == hipSuccess) e = hipMalloc(&c, view * 4);
+  if (e == hipSuccess) e = hipMalloc(&h, sizeof(float) * std::max(N, 1));
+  if (e == hipSuccess) e = hipMemcpy(s, host_src, view * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c, host_cos, view * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h, host_filter, sizeof(float) * N, hipMemcpyHostToDevice);
+  int r = e == hipSuccess ? FDK_OK : fdk::fail(FDK_ERR_HIP, "FilterEngine upload: %s", hipGetErrorString(e));
+  if (r == FDK_OK)
+    r = fdk_filter_views(s, (int64_t)view, d, (int64_t)view, FDK_F32, nu, nv, 1, c, h, N, pad_mode,
+                         transpose, nullptr, nullptr);
+  if (r == FDK_OK) r = fdk_correct_image(s, c, s, nu, nv, 1, nu, nu, nullptr);
+  if (r == FDK_OK) {
+    e = hipMemcpy(host_dst, d, view * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(host_src, s, view * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fdk::fail(FDK_ERR_HIP, "FilterEngine download: %s", hipGetErrorString(e));
+  }
+  if (s) (void)hipFree(s);
+  if (d) (void)hipFree(d);
+  if (c) (void)hipFree(c);
+  if (h) (void)hipFree(h);
+  return r;
+}

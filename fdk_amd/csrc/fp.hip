@@ -276,3 +276,24 @@ extern "C" int fdk_phantom_project(const fdk_ellipsoid* host_ell, int n, const f
   if (e != hipSuccess) return fdk::fail(FDK_ERR_HIP, "phantom_project: %s", hipGetErrorString(e));
   return FDK_OK;
 }
+
+extern "C" int fdk_fp_project_host(const fdk_geometry* g, const float* host_vol, int first,
+                                   int count, double step_mm, int mode, float* host_out) {
+  FDK_REQUIRE(g && host_vol && host_out, "NULL pointer");
+  FDK_REQUIRE(count >= 0, "bad view count");
+  const size_t nvox = (size_t)g->nx * g->ny * g->nz;
+  const size_t nout = (size_t)g->nu * g->nv * count;
+  float *v = nullptr, *o = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<size_t>(nvox, 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&o, std::max<size_t>(nout, 1) * 4);
+  if (e == hipSuccess) e = hipMemcpy(v, host_vol, nvox * 4, hipMemcpyHostToDevice);
+  int r = e == hipSuccess ? FDK_OK : fdk::fail(FDK_ERR_HIP, "forward projection upload: %s", hipGetErrorString(e));
+  if (r == FDK_OK) r = fdk_fp_project(g, v, first, count, step_mm, mode, o, nullptr);
+  if (r == FDK_OK) {
+    e = hipMemcpy(host_out, o, nout * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fdk::fail(FDK_ERR_HIP, "forward projection download: %s", hipGetErrorString(e));
+  }
+  if (v) (void)hipFree(v);
+  if (o) (void)hipFree(o);
+  return r;
+}

@@ -196,12 +196,23 @@ int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst
 int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nviews, const float* host_cos,
                           const float* host_filter, int N, int pad_mode, int depth);
 
+/* FilterEngine::Filtering on one host image (nv rows of nu): src is overwritten
+ * with the cos-weighted image (FilterLib.h:181), dst receives the filtered
+ * image, transposed (nu rows of nv) when transpose is 1.  Synchronous. */
+int fdk_filter_frame_host(float* host_src, float* host_dst, int nu, int nv, const float* host_cos,
+                          const float* host_filter, int N, int pad_mode, int transpose);
+
 /* ------------------------------------------------------------ forward projection */
 /* Ray-marching projection of views [first, first+count) of geometry g through
  * dev_vol (nz x ny x nx floats, x fastest; sizes and spacing from g), step_mm
  * (<= 0: min(dx,dy,dz), the reference's step).  dev_out: count x nv x nu floats. */
 int fdk_fp_project(const fdk_geometry* g, const float* dev_vol, int first, int count,
                    double step_mm, int mode, float* dev_out, void* stream);
+
+/* Host-memory form of fdk_fp_project (the volume is uploaded, views come back
+ * to host_out); synchronous. */
+int fdk_fp_project_host(const fdk_geometry* g, const float* host_vol, int first, int count,
+                        double step_mm, int mode, float* host_out);
 
 /* ------------------------------------------------------------ random phantoms */
 /* Voxel-centre rasterization into dev_vol (nz x ny x nx floats, overwritten). */
