@@ -1,0 +1,730 @@
+// Kernels of the learned sparse-view correction: a 3-D residual U-Net on
+// BF16 MFMA (gfx950 v_mfma_f32_16x16x32_bf16), FP32 accumulation.  No reference
+// counterpart: the reference has no learned component.
+//
+// Activations are channels-last, [n][z][y][x][c] BF16.  A 3x3x3 'same'
+// convolution is an implicit GEMM: M = output voxels, N = output channels,
+// K = 27 taps x input channels.  With channels innermost, the 8 consecutive K
+// values a lane feeds one MFMA are 16 contiguous bytes of one voxel (A) or of
+// one weight row (B), so operands come straight from L1/L2 as 16-byte loads.
+//
+// MFMA 16x16x32 BF16 lane layout (checked by fdk_nn_mfma_probe's test):
+//   A (16 x 32): lane l holds A[l % 16][8 (l / 16) + j], j = 0..7
+//   B (32 x 16): lane l holds B[8 (l / 16) + j][l % 16]
+//   D (16 x 16): lane l holds D[4 (l / 16) + i][l % 16], i = 0..3
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+__device__ inline float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline u16 f2bf(float f) {  // round to nearest even
+  const unsigned b = __float_as_uint(f);
+  if ((b & 0x7f800000u) == 0x7f800000u) return (u16)(b >> 16);  // inf / nan
+  return (u16)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 load8(const u16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+  for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+  return z;
+}
+
+struct Conv {
+  const u16* x;      // [n][d][h][w][cin]
+  const u16* wt;     // [27][cout][cin] (cin == 1: [cout][32] taps, zero padded)
+  const float* bias; // [cout] or null
+  const float* resid;// [n][d][h][w][cout] FP32 added to the output, or null
+  void* y;           // [n][d][h][w][cout], BF16 or FP32
+  int n, d, h, w, cin, cout;
+  int relu, out_f32;
+};
+
+template <int NT>
+__device__ inline void conv_epilogue(const Conv& a, f32x4 (&acc)[2][NT], int nzy, int x0,
+                                     int lane) {
+  const int m = lane % 16, kq = lane / 16;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + mt * 16 + 4 * kq + i;
+        const int co = nt * 16 + m;
+        const long long o = ((long long)nzy * a.w + x) * a.cout + co;
+        float v = acc[mt][nt][i];
+        if (a.bias) v += a.bias[co];
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.resid) v += a.resid[o];
+        if (a.out_f32) static_cast<float*>(a.y)[o] = v;
+        else static_cast<u16*>(a.y)[o] = f2bf(v);
+      }
+}
+
+// One wave: 32 output voxels of one x-row (two 16-voxel M tiles) x all cout
+// (NT tiles of 16).  Block: 4 waves = 4 consecutive rows.  cin % 32 == 0.
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd_kernel(Conv a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int x0 = blockIdx.x * 32;
+  const int y = blockIdx.y * 4 + wave;
+  const int nz = blockIdx.z;  // n * d + z
+  const int z = nz % a.d, n = nz / a.d;
+  if (y >= a.h) return;
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int zz = z + dz, yy = y + dy;
+    const bool zy = zz >= 0 && zz < a.d && yy >= 0 && yy < a.h;
+    const long long rowbase = (((long long)n * a.d + zz) * a.h + yy) * a.w;
+    for (int c0 = 0; c0 < a.cin; c0 += 32) {
+      bf16x8 af[2];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        const int xx = x0 + mt * 16 + m + dx;
+        af[mt] = (zy && xx >= 0 && xx < a.w) ? load8(a.x + (rowbase + xx) * a.cin + c0 + kq * 8)
+                                            : zero8();
+      }
+      const u16* wb = a.wt + (long long)tap * a.cout * a.cin + c0 + kq * 8;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const bf16x8 bf = load8(wb + (long long)(nt * 16 + m) * a.cin);
+        acc[0][nt] = mfma(af[0], bf, acc[0][nt]);
+        acc[1][nt] = mfma(af[1], bf, acc[1][nt]);
+      }
+    }
+  }
+  conv_epilogue<NT>(a, acc, nz * a.h + y, x0, lane);
+}
+
+// cin == 1 (the network input): the 27 taps of the single channel form K (one
+// MFMA K step, padded to 32); weights [cout][32].
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd_c1_kernel(Conv a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int x0 = blockIdx.x * 32;
+  const int y = blockIdx.y * 4 + wave;
+  const int nz = blockIdx.z;
+  const int z = nz % a.d, n = nz / a.d;
+  if (y >= a.h) return;
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  bf16x8 af[2];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    const int x = x0 + mt * 16 + m;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int tap = kq * 8 + j;
+      float v = 0.f;
+      if (tap < 27) {
+        const int zz = z + tap / 9 - 1, yy = y + (tap / 3) % 3 - 1, xx = x + tap % 3 - 1;
+        if (zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+          v = bf2f(a.x[(((long long)n * a.d + zz) * a.h + yy) * a.w + xx]);
+      }
+      af[mt][j] = (__bf16)v;
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const bf16x8 bf = load8(a.wt + (nt * 16 + m) * 32 + kq * 8);
+    acc[0][nt] = mfma(af[0], bf, acc[0][nt]);
+    acc[1][nt] = mfma(af[1], bf, acc[1][nt]);
+  }
+  conv_epilogue<NT>(a, acc, nz * a.h + y, x0, lane);
+}
+
+// Weight gradient dW[tap][co][ci] += sum over voxels dY[v][co] X[v + tap][ci]
+// (K = voxels).  A block owns one tap and a run of `rows` output rows (n, z, y);
+// per row it stages dY and the tap-shifted X row in LDS transposed to [c][x] so
+// each lane's 8 consecutive voxels are one 16-byte LDS read, accumulates all
+// (co, ci) 16x16 tiles split over its 4 waves, and adds them to dW once.
+constexpr int kWgradRows = 64;
+
+struct Wgrad {
+  const u16* x;   // [n][d][h][w][cin]
+  const u16* dy;  // [n][d][h][w][cout]
+  float* dw;      // [27][cout][cin], accumulated with atomics
+  int n, d, h, w, cin, cout;
+};
+
+template <int MAXT>
+__global__ __launch_bounds__(256) void conv3d_wgrad_kernel(Wgrad a) {
+  extern __shared__ u16 lds[];
+  const int W = a.w, WP = W + 8;   // padded row length in LDS
+  u16* dyT = lds;                  // [cout][WP]
+  u16* xT = lds + a.cout * WP;     // [cin][WP]
+  const int tap = blockIdx.y;
+  const int dz = tap / 9 - 1, dy_ = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int tco = a.cout / 16, tci = a.cin / 16, ntiles = tco * tci;
+  f32x4 acc[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int total_rows = a.n * a.d * a.h;
+  const int r0 = blockIdx.x * kWgradRows;
+  for (int r = r0; r < min(r0 + kWgradRows, total_rows); ++r) {
+    const int y = r % a.h, z = (r / a.h) % a.d, n = r / (a.h * a.d);
+    const int zz = z + dz, yy = y + dy_;
+    const bool zy = zz >= 0 && zz < a.d && yy >= 0 && yy < a.h;
+    __syncthreads();
+    const u16* dyrow = a.dy + (long long)r * W * a.cout;
+    for (int i = threadIdx.x; i < W * a.cout; i += 256) {
+      const int x = i / a.cout, c = i % a.cout;
+      dyT[c * WP + x] = dyrow[i];
+    }
+    const u16* xrow = a.x + ((((long long)n * a.d + zz) * a.h + yy) * W) * a.cin;
+    for (int i = threadIdx.x; i < W * a.cin; i += 256) {
+      const int x = i / a.cin, c = i % a.cin;
+      const int xs = x + dx;
+      xT[c * WP + x] = (zy && xs >= 0 && xs < W) ? xrow[(long long)xs * a.cin + c] : (u16)0;
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < W; k0 += 32) {
+      int t = 0;
+      for (int tile = wave; tile < ntiles; tile += 4, ++t) {
+        const int co = (tile / tci) * 16 + m, ci = (tile % tci) * 16 + m;
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(dyT + co * WP + k0 + kq * 8);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(xT + ci * WP + k0 + kq * 8);
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q)
+          if (q == t) acc[q] = mfma(av, bv, acc[q]);
+      }
+    }
+  }
+  int t = 0;
+  for (int tile = wave; tile < ntiles; tile += 4, ++t) {
+    const int co0 = (tile / tci) * 16, ci = (tile % tci) * 16 + m;
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q)
+      if (q == t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int co = co0 + 4 * kq + i;
+          atomicAdd(&a.dw[((long long)tap * a.cout + co) * a.cin + ci], acc[q][i]);
+        }
+  }
+}
+
+// Weight gradient of the cin == 1 convolution: dW[co][tap] (tap < 27, row
+// length 32) += sum_v dY[v][co] X[v + tap].  FP32 vector math.
+__global__ __launch_bounds__(256) void conv3d_wgrad_c1_kernel(Wgrad a) {
+  const int co = threadIdx.x % 32;
+  const int sub = threadIdx.x / 32;  // 8 voxel lanes per channel
+  float acc[27];
+  for (int t = 0; t < 27; ++t) acc[t] = 0.f;
+  const long long nvox = (long long)a.n * a.d * a.h * a.w;
+  for (long long v = (long long)blockIdx.x * 8 + sub; v < nvox; v += (long long)gridDim.x * 8) {
+    const int x = v % a.w, y = (v / a.w) % a.h, z = (v / ((long long)a.w * a.h)) % a.d;
+    const long long n = v / ((long long)a.w * a.h * a.d);
+    const float g = bf2f(a.dy[v * a.cout + co]);
+    if (g == 0.f) continue;
+    for (int t = 0; t < 27; ++t) {
+      const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+      if (zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+        acc[t] += g * bf2f(a.x[((n * a.d + zz) * a.h + yy) * a.w + xx]);
+    }
+  }
+  for (int t = 0; t < 27; ++t) atomicAdd(&a.dw[co * 32 + t], acc[t]);
+}
+
+// dW of a [27][cout][cin] forward weight -> backward-data weight
+// Wb[t][ci][co] = W[26 - t][co][ci] (spatial flip, channel transpose).
+__global__ void flip_weights_kernel(const u16* w, u16* wb, int cin, int cout) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= 27LL * cin * cout) return;
+  const int co = i % cout, ci = (i / cout) % cin, t = i / ((long long)cout * cin);
+  wb[i] = w[((26 - t) * (long long)cout + co) * cin + ci];
+}
+
+__global__ void bias_grad_kernel(const u16* dy, float* db, long long nvox, int c) {
+  const int ch = threadIdx.x % c;
+  const int lanes = 256 / c;
+  const int sub = threadIdx.x / c;
+  if (sub >= lanes) return;
+  float s = 0.f;
+  for (long long v = (long long)blockIdx.x * lanes + sub; v < nvox; v += (long long)gridDim.x * lanes)
+    s += bf2f(dy[v * c + ch]);
+  atomicAdd(&db[ch], s);
+}
+
+__global__ void relu_bwd_kernel(const u16* dy, const u16* y, u16* dz, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dz[i] = (bf2f(y[i]) > 0.f) ? dy[i] : (u16)0;
+}
+
+// 2x2x2 average pooling (and its adjoint) on [n][d][h][w][c].
+__global__ void avgpool_kernel(const u16* x, u16* y, int n, int d, int h, int w, int c) {
+  const long long total = (long long)n * (d / 2) * (h / 2) * (w / 2) * c;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int ch = i % c;
+    long long v = i / c;
+    const int xo = v % (w / 2); v /= (w / 2);
+    const int yo = v % (h / 2); v /= (h / 2);
+    const int zo = v % (d / 2);
+    const long long nn = v / (d / 2);
+    float s = 0.f;
+    for (int q = 0; q < 8; ++q) {
+      const int zz = 2 * zo + (q >> 2), yy = 2 * yo + ((q >> 1) & 1), xx = 2 * xo + (q & 1);
+      s += bf2f(x[(((nn * d + zz) * h + yy) * w + xx) * c + ch]);
+    }
+    y[i] = f2bf(0.125f * s);
+  }
+}
+
+__global__ void avgpool_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, int w, int c) {
+  const long long total = (long long)n * d * h * w * c;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int ch = i % c;
+    long long v = i / c;
+    const int x = v % w; v /= w;
+    const int y = v % h; v /= h;
+    const int z = v % d;
+    const long long nn = v / d;
+    const long long o = (((nn * (d / 2) + z / 2) * (h / 2) + y / 2) * (w / 2) + x / 2) * c + ch;
+    dx[i] = f2bf(0.125f * bf2f(dy[o]));
+  }
+}
+
+// Nearest 2x upsampling of x ([n][d][h][w][cx]) into channels [c0, c0+cx) of
+// y ([n][2d][2h][2w][cy]); the adjoint sums the 8 children.
+__global__ void upsample_kernel(const u16* x, u16* y, int n, int d, int h, int w, int cx, int cy,
+                                int c0) {
+  const long long total = (long long)n * 8 * d * h * w * cx;
+  const int D = 2 * d, H = 2 * h, Wd = 2 * w;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int ch = i % cx;
+    long long v = i / cx;
+    const int X = v % Wd; v /= Wd;
+    const int Y = v % H; v /= H;
+    const int Z = v % D;
+    const long long nn = v / D;
+    y[(((nn * D + Z) * H + Y) * Wd + X) * cy + c0 + ch] =
+        x[(((nn * d + Z / 2) * h + Y / 2) * w + X / 2) * cx + ch];
+  }
+}
+
+__global__ void upsample_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, int w, int cx,
+                                    int cy, int c0) {
+  const long long total = (long long)n * d * h * w * cx;
+  const int D = 2 * d, H = 2 * h, Wd = 2 * w;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int ch = i % cx;
+    long long v = i / cx;
+    const int x = v % w; v /= w;
+    const int y = v % h; v /= h;
+    const int z = v % d;
+    const long long nn = v / d;
+    float s = 0.f;
+    for (int q = 0; q < 8; ++q) {
+      const int Z = 2 * z + (q >> 2), Y = 2 * y + ((q >> 1) & 1), X = 2 * x + (q & 1);
+      s += bf2f(dy[(((nn * D + Z) * H + Y) * Wd + X) * cy + c0 + ch]);
+    }
+    dx[i] = f2bf(s);
+  }
+}
+
+// Copy channels [src_c0, src_c0+cc) of a [nvox][cs] tensor into channels
+// [dst_c0, dst_c0+cc) of a [nvox][cd] tensor (concatenation and its adjoint).
+__global__ void channel_copy_kernel(const u16* src, int cs, int src_c0, u16* dst, int cd,
+                                    int dst_c0, int cc, long long nvox) {
+  const long long total = nvox * cc;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const long long v = i / cc;
+    const int ch = i % cc;
+    dst[v * cd + dst_c0 + ch] = src[v * cs + src_c0 + ch];
+  }
+}
+
+// Output head: out = in + sum_c w[c] h[c] + b (FP32), and its adjoints.
+__global__ void head_fwd_kernel(const u16* h, const float* w, const float* b, const float* in,
+                                float* out, long long nvox, int c) {
+  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long long)gridDim.x * 256) {
+    float s = b[0];
+    for (int ch = 0; ch < c; ++ch) s = fmaf(w[ch], bf2f(h[v * c + ch]), s);
+    out[v] = in[v] + s;
+  }
+}
+
+__global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w, u16* dh,
+                                float* dw, float* db, long long nvox, int c) {
+  __shared__ float red[64];
+  for (int i = threadIdx.x; i < 64; i += 256) red[i] = 0.f;
+  __syncthreads();
+  float acc[64];
+  for (int ch = 0; ch < c; ++ch) acc[ch] = 0.f;
+  float accb = 0.f;
+  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long long)gridDim.x * 256) {
+    const float g = dout[v];
+    accb += g;
+    for (int ch = 0; ch < c; ++ch) {
+      acc[ch] = fmaf(g, bf2f(h[v * c + ch]), acc[ch]);
+      dh[v * c + ch] = f2bf(g * w[ch]);
+    }
+  }
+  for (int ch = 0; ch < c; ++ch) atomicAdd(&red[ch], acc[ch]);
+  atomicAdd(&red[c], accb);
+  __syncthreads();
+  for (int i = threadIdx.x; i < c; i += 256) atomicAdd(&dw[i], red[i]);
+  if (threadIdx.x == 0) atomicAdd(db, red[c]);
+}
+
+// Mean-squared error: loss_sum += sum (out - t)^2, dout = 2 (out - t) / nvox.
+__global__ void mse_kernel(const float* out, const float* t, float* dout, float* loss_sum,
+                           long long nvox) {
+  float s = 0.f;
+  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long long)gridDim.x * 256) {
+    const float e = out[v] - t[v];
+    s += e * e;
+    if (dout) dout[v] = 2.0f * e / (float)nvox;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, s);
+}
+
+// Adam on FP32 master weights; also refreshes the BF16 copy the MFMA reads
+// (layout given by `bf_stride`: the BF16 copy may pad rows, e.g. the cin == 1
+// conv's 27 taps padded to 32; index i maps to (i / row) * bf_row + i % row).
+__global__ void adam_kernel(float* w, u16* wbf, const float* g, float* m, float* v, long long n,
+                            float lr, float b1, float b2, float eps, float c1, float c2, int row,
+                            int bf_row) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wbf) wbf[(i / row) * bf_row + i % row] = f2bf(wi);
+  }
+}
+
+__global__ void f32_to_bf16_kernel(const float* s, u16* d, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    d[i] = f2bf(s[i]);
+}
+
+__global__ void bf16_to_f32_kernel(const u16* s, float* d, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    d[i] = bf2f(s[i]);
+}
+
+// Box copy between [D][H][W] FP32 volumes: dst[z][y][x] = src[z+sz][y+sy][x+sx]
+// for the (nz, ny, nx) box; a channel-1 activation is the same array.
+__global__ void box_copy_kernel(const float* src, int sD, int sH, int sW, int sz, int sy, int sx,
+                                float* dst, int dD, int dH, int dW, int dz, int dy, int dx, int nz,
+                                int ny, int nx, float scale) {
+  const long long total = (long long)nz * ny * nx;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int x = i % nx, y = (i / nx) % ny, z = i / ((long long)nx * ny);
+    dst[((long long)(z + dz) * dH + (y + dy)) * dW + (x + dx)] =
+        scale * src[((long long)(z + sz) * sH + (y + sy)) * sW + (x + sx)];
+  }
+}
+
+// Photon noise on line integrals: counts ~ Poisson(I0 exp(-mu p)),
+// p <- -ln(max(counts, 1) / I0) / mu.  Counter-based hash RNG per (seed, i);
+// Poisson by inversion below 64 expected counts, normal approximation above.
+__device__ inline unsigned hash32(unsigned x) {
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+__device__ inline float unif(unsigned seed, unsigned long long i, unsigned k) {
+  const unsigned h = hash32(hash32(seed ^ (unsigned)(i * 0x9E3779B97F4A7C15ULL >> 32)) ^
+                            hash32((unsigned)i + 0x632BE5ABu * k));
+  return ((h >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+__global__ void poisson_kernel(float* p, long long n, float i0, float mu, unsigned seed) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const float lam = i0 * expf(-mu * p[i]);
+    float k;
+    if (lam < 64.f) {
+      const float L = expf(-lam);
+      float prod = unif(seed, i, 0);
+      int c = 0;
+      while (prod > L && c < 1000) {
+        ++c;
+        prod *= unif(seed, i, (unsigned)c);
+      }
+      k = (float)c;
+    } else {
+      const float u1 = unif(seed, i, 1), u2 = unif(seed, i, 2);
+      const float g = sqrtf(-2.f * logf(u1)) * cosf(6.28318530718f * u2);
+      k = rintf(lam + sqrtf(lam) * g);
+    }
+    p[i] = -logf(fmaxf(k, 1.f) / i0) / mu;
+  }
+}
+
+// Layout probe of v_mfma_f32_16x16x32_bf16: D = A (16x32) * B (32x16) with
+// A, B given row-major in FP32 (rounded to BF16), D row-major.
+__global__ void mfma_probe_kernel(const float* A, const float* B, float* D) {
+  const int l = threadIdx.x, m = l % 16, kq = l / 16;
+  bf16x8 a, b;
+  for (int j = 0; j < 8; ++j) {
+    a[j] = (__bf16)A[m * 32 + kq * 8 + j];
+    b[j] = (__bf16)B[(kq * 8 + j) * 16 + m];
+  }
+  const f32x4 d = mfma(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
+  for (int i = 0; i < 4; ++i) D[(4 * kq + i) * 16 + m] = d[i];
+}
+
+int blocks_for(long long n) { return (int)std::min<long long>(4096, std::max<long long>(1, (n + 255) / 256)); }
+
+}  // namespace
+
+extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void* stream) {
+  FDK_REQUIRE(A && B && D, "NULL pointer");
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), A, B, D);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid,
+                                 void* y, int n, int d, int h, int wd, int cin, int cout, int relu,
+                                 int out_f32, void* stream) {
+  FDK_REQUIRE(x && w && y, "NULL pointer");
+  FDK_REQUIRE(n > 0 && d > 0 && h > 0 && wd > 0, "bad tensor size");
+  FDK_REQUIRE(wd % 32 == 0 && h % 4 == 0, "conv3d needs W %% 32 == 0 and H %% 4 == 0 (got %d, %d)", wd, h);
+  FDK_REQUIRE(cin == 1 || cin % 32 == 0, "cin must be 1 or a multiple of 32 (got %d)", cin);
+  FDK_REQUIRE((long long)n * d <= 65535, "n * d exceeds the grid");
+  Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
+         cin, cout, relu, out_f32};
+  dim3 grid(wd / 32, h / 4, n * d);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define FDK_CONV_CASE(NT)                                                                   \
+  case NT:                                                                                  \
+    if (cin == 1) hipLaunchKernelGGL(conv3d_fwd_c1_kernel<NT>, grid, dim3(256), 0, st, a);  \
+    else hipLaunchKernelGGL(conv3d_fwd_kernel<NT>, grid, dim3(256), 0, st, a);              \
+    break;
+  switch (cout / 16) {
+    FDK_CONV_CASE(2)
+    FDK_CONV_CASE(4)
+    FDK_CONV_CASE(6)
+    FDK_CONV_CASE(8)
+    FDK_CONV_CASE(12)
+    default:
+      return fdk::fail(FDK_ERR_UNSUPPORTED, "cout %d not in {32, 64, 96, 128, 192}", cout);
+  }
+#undef FDK_CONV_CASE
+  if (cout % 16) return fdk::fail(FDK_ERR_UNSUPPORTED, "cout %d not a multiple of 16", cout);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h,
+                                   int wd, int cin, int cout, void* stream) {
+  FDK_REQUIRE(x && dy && dw, "NULL pointer");
+  FDK_REQUIRE(cout % 16 == 0 && cout <= 256, "cout must be a multiple of 16 up to 256");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Wgrad a{static_cast<const u16*>(x), static_cast<const u16*>(dy), dw, n, d, h, wd, cin, cout};
+  if (cin == 1) {
+    FDK_REQUIRE(cout == 32, "the cin == 1 weight gradient takes cout == 32");
+    hipLaunchKernelGGL(conv3d_wgrad_c1_kernel, dim3(2048), dim3(256), 0, st, a);
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
+  FDK_REQUIRE(cin % 16 == 0 && wd % 32 == 0, "cin %% 16 and W %% 32 must be 0");
+  const int ntiles = (cout / 16) * (cin / 16);
+  const int per_wave = (ntiles + 3) / 4;
+  const size_t lds = (size_t)(cout + cin) * (wd + 8) * sizeof(u16);
+  FDK_REQUIRE(lds <= 160 * 1024, "weight-gradient tile needs %zu bytes of LDS", lds);
+  const int rows = n * d * h;
+  dim3 grid((rows + kWgradRows - 1) / kWgradRows, 27);
+  if (per_wave <= 4) hipLaunchKernelGGL(conv3d_wgrad_kernel<4>, grid, dim3(256), lds, st, a);
+  else if (per_wave <= 8) hipLaunchKernelGGL(conv3d_wgrad_kernel<8>, grid, dim3(256), lds, st, a);
+  else if (per_wave <= 16) hipLaunchKernelGGL(conv3d_wgrad_kernel<16>, grid, dim3(256), lds, st, a);
+  else if (per_wave <= 24) hipLaunchKernelGGL(conv3d_wgrad_kernel<24>, grid, dim3(256), lds, st, a);
+  else return fdk::fail(FDK_ERR_UNSUPPORTED, "too many weight tiles (%d)", ntiles);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_flip_weights(const void* w, void* wb, int cin, int cout, void* stream) {
+  FDK_REQUIRE(w && wb, "NULL pointer");
+  const long long n = 27LL * cin * cout;
+  hipLaunchKernelGGL(flip_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const u16*>(w), static_cast<u16*>(wb), cin, cout);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_bias_grad(const void* dy, float* db, int64_t nvox, int c, void* stream) {
+  FDK_REQUIRE(dy && db && c > 0 && c <= 256 && 256 % c == 0, "bias_grad needs c dividing 256");
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(1024), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const u16*>(dy), db, (long long)nvox, c);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, void* stream) {
+  FDK_REQUIRE(dy && y && dz, "NULL pointer");
+  hipLaunchKernelGGL(relu_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const u16*>(dy), static_cast<const u16*>(y), static_cast<u16*>(dz), (long long)n);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_avgpool2(const void* x, void* y, int n, int d, int h, int w, int c, int adjoint,
+                               void* stream) {
+  FDK_REQUIRE(x && y && d % 2 == 0 && h % 2 == 0 && w % 2 == 0, "avgpool needs even sizes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long full = (long long)n * d * h * w * c;
+  if (!adjoint)
+    hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for(full / 8)), dim3(256), 0, st,
+                       static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, c);
+  else
+    hipLaunchKernelGGL(avgpool_bwd_kernel, dim3(blocks_for(full)), dim3(256), 0, st,
+                       static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, c);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_upsample2(const void* x, void* y, int n, int d, int h, int w, int cx, int cy,
+                                int c0, int adjoint, void* stream) {
+  FDK_REQUIRE(x && y && c0 >= 0 && c0 + cx <= cy, "bad channel range");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long small = (long long)n * d * h * w * cx;
+  if (!adjoint)
+    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(small * 8)), dim3(256), 0, st,
+                       static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, cx, cy, c0);
+  else
+    hipLaunchKernelGGL(upsample_bwd_kernel, dim3(blocks_for(small)), dim3(256), 0, st,
+                       static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, cx, cy, c0);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_channel_copy(const void* src, int cs, int src_c0, void* dst, int cd,
+                                   int dst_c0, int cc, int64_t nvox, void* stream) {
+  FDK_REQUIRE(src && dst && src_c0 + cc <= cs && dst_c0 + cc <= cd, "bad channel range");
+  hipLaunchKernelGGL(channel_copy_kernel, dim3(blocks_for(nvox * cc)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const u16*>(src), cs, src_c0,
+                     static_cast<u16*>(dst), cd, dst_c0, cc, (long long)nvox);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_head(const void* h, const float* w, const float* b, const float* in,
+                           float* out, int64_t nvox, int c, void* stream) {
+  FDK_REQUIRE(h && w && b && in && out && c <= 63, "bad head arguments");
+  hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(nvox)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const u16*>(h), w, b, in, out,
+                     (long long)nvox, c);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_head_bwd(const float* dout, const void* h, const float* w, void* dh,
+                               float* dw, float* db, int64_t nvox, int c, void* stream) {
+  FDK_REQUIRE(dout && h && w && dh && dw && db && c <= 63, "bad head arguments");
+  hipLaunchKernelGGL(head_bwd_kernel, dim3(std::min<long long>(1024, blocks_for(nvox))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), dout, static_cast<const u16*>(h), w,
+                     static_cast<u16*>(dh), dw, db, (long long)nvox, c);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_mse(const float* out, const float* target, float* dout, float* loss_sum,
+                          int64_t nvox, void* stream) {
+  FDK_REQUIRE(out && target && loss_sum, "NULL pointer");
+  hipLaunchKernelGGL(mse_kernel, dim3(std::min<long long>(2048, blocks_for(nvox))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), out, target, dout, loss_sum, (long long)nvox);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_adam(float* w, void* wbf, const float* g, float* m, float* v, int64_t n,
+                           float lr, float b1, float b2, float eps, int step, int row, int bf_row,
+                           void* stream) {
+  FDK_REQUIRE(w && g && m && v && step >= 1 && row >= 1 && bf_row >= row, "bad Adam arguments");
+  const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     w, static_cast<u16*>(wbf), g, m, v, (long long)n, lr, b1, b2, eps, c1, c2, row, bf_row);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_convert(const void* src, void* dst, int64_t n, int to_bf16, void* stream) {
+  FDK_REQUIRE(src && dst, "NULL pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (to_bf16)
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, st,
+                       static_cast<const float*>(src), static_cast<u16*>(dst), (long long)n);
+  else
+    hipLaunchKernelGGL(bf16_to_f32_kernel, dim3(blocks_for(n)), dim3(256), 0, st,
+                       static_cast<const u16*>(src), static_cast<float*>(dst), (long long)n);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_box_copy(const float* src, const int src_dims[3], const int src_org[3],
+                               float* dst, const int dst_dims[3], const int dst_org[3],
+                               const int box[3], float scale, void* stream) {
+  FDK_REQUIRE(src && dst && src_dims && src_org && dst_dims && dst_org && box, "NULL pointer");
+  for (int e = 0; e < 3; ++e) {
+    FDK_REQUIRE(box[e] >= 0 && src_org[e] >= 0 && dst_org[e] >= 0, "negative box");
+    FDK_REQUIRE(src_org[e] + box[e] <= src_dims[e] && dst_org[e] + box[e] <= dst_dims[e],
+                "box leaves a volume on axis %d", e);
+  }
+  const long long n = (long long)box[0] * box[1] * box[2];
+  if (n == 0) return FDK_OK;
+  hipLaunchKernelGGL(box_copy_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     src, src_dims[0], src_dims[1], src_dims[2], src_org[0], src_org[1], src_org[2],
+                     dst, dst_dims[0], dst_dims[1], dst_dims[2], dst_org[0], dst_org[1], dst_org[2],
+                     box[0], box[1], box[2], scale);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t seed,
+                                 void* stream) {
+  FDK_REQUIRE(dev_proj && i0 > 0 && mu > 0, "bad noise arguments");
+  hipLaunchKernelGGL(poisson_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     dev_proj, (long long)n, (float)i0, (float)mu, seed);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+namespace {
+__global__ void add_bf16_kernel(const u16* a, const u16* b, u16* y, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    y[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+}  // namespace
+
+extern "C" int fdk_nn_add(const void* a, const void* b, void* y, int64_t n, void* stream) {
+  FDK_REQUIRE(a && b && y, "NULL pointer");
+  hipLaunchKernelGGL(add_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const u16*>(a), static_cast<const u16*>(b), static_cast<u16*>(y), (long long)n);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}

@@ -222,6 +222,62 @@ int fdk_phantom_rasterize(const fdk_ellipsoid* host_ell, int n, const fdk_geomet
 int fdk_phantom_project(const fdk_ellipsoid* host_ell, int n, const fdk_geometry* g, int first,
                         int count, float* dev_out, void* stream);
 
+/* ------------------------------------------------------------ learned correction */
+/* Kernels of the 3-D residual U-Net (no reference counterpart: the reference has
+ * no learned component).  Activations are channels-last [n][z][y][x][c] BF16
+ * (uint16 storage); weights BF16 for the MFMA, FP32 masters for Adam.
+ * A 3x3x3 'same' convolution with zero padding: weights [27][cout][cin]
+ * (tap = 9 dz + 3 dy + dx over -1..1), or [cout][32] taps for cin == 1.
+ * y = relu?(conv(x) + bias) + resid, stored BF16 or FP32 (out_f32).
+ * Needs W % 32 == 0, H % 4 == 0, cin == 1 or cin % 32 == 0, cout in
+ * {32, 64, 96, 128, 192}. */
+int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid, void* y,
+                      int n, int d, int h, int w_, int cin, int cout, int relu, int out_f32,
+                      void* stream);
+/* dW[27][cout][cin] (FP32, accumulated) += sum_v dY[v][co] X[v + tap][ci];
+ * cin == 1 accumulates into [cout][32]. */
+int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h, int w_,
+                        int cin, int cout, void* stream);
+/* Backward-data weights: wb[t][ci][co] = w[26 - t][co][ci]; the input gradient
+ * is then fdk_nn_conv3d_fwd(dy, wb, ..., cin = cout, cout = cin). */
+int fdk_nn_flip_weights(const void* w, void* wb, int cin, int cout, void* stream);
+int fdk_nn_bias_grad(const void* dy, float* db, int64_t nvox, int c, void* stream);
+int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, void* stream);
+/* 2x2x2 average pooling of [n][d][h][w][c] (adjoint 1: [n][d/2][h/2][w/2][c] -> full). */
+int fdk_nn_avgpool2(const void* x, void* y, int n, int d, int h, int w, int c, int adjoint,
+                    void* stream);
+/* Nearest 2x upsampling of [n][d][h][w][cx] into channels [c0, c0+cx) of a
+ * [n][2d][2h][2w][cy] tensor (adjoint 1: the reverse, summing the 8 children). */
+int fdk_nn_upsample2(const void* x, void* y, int n, int d, int h, int w, int cx, int cy, int c0,
+                     int adjoint, void* stream);
+int fdk_nn_channel_copy(const void* src, int cs, int src_c0, void* dst, int cd, int dst_c0,
+                        int cc, int64_t nvox, void* stream);
+int fdk_nn_add(const void* a, const void* b, void* y, int64_t n, void* stream);
+/* Output head: out = in + sum_c w[c] h[v][c] + b (FP32), and its adjoint. */
+int fdk_nn_head(const void* h, const float* w, const float* b, const float* in, float* out,
+                int64_t nvox, int c, void* stream);
+int fdk_nn_head_bwd(const float* dout, const void* h, const float* w, void* dh, float* dw,
+                    float* db, int64_t nvox, int c, void* stream);
+/* loss_sum += sum (out - t)^2; dout = 2 (out - t) / nvox when dout != NULL. */
+int fdk_nn_mse(const float* out, const float* target, float* dout, float* loss_sum, int64_t nvox,
+               void* stream);
+/* Adam step (bias-corrected) on FP32 weights; refreshes the BF16 copy when wbf != NULL
+ * (element i -> (i / row) * bf_row + i % row). */
+int fdk_nn_adam(float* w, void* wbf, const float* g, float* m, float* v, int64_t n, float lr,
+                float b1, float b2, float eps, int step, int row, int bf_row, void* stream);
+int fdk_nn_convert(const void* src, void* dst, int64_t n, int to_bf16, void* stream);
+/* dst[dst_org + i] = scale * src[src_org + i] over a box of FP32 [D][H][W] volumes. */
+int fdk_nn_box_copy(const float* src, const int src_dims[3], const int src_org[3], float* dst,
+                    const int dst_dims[3], const int dst_org[3], const int box[3], float scale,
+                    void* stream);
+/* Photon noise on line integrals p: counts ~ Poisson(i0 exp(-mu p)), then
+ * p = -ln(max(counts, 1) / i0) / mu. */
+int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t seed,
+                      void* stream);
+/* D (16x16) = A (16x32) B (32x16) through one v_mfma_f32_16x16x32_bf16 (row-major
+ * FP32 in, BF16-rounded operands); a lane-layout check. */
+int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
