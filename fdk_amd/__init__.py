@@ -414,18 +414,21 @@ def FilterRow_Real32(src, dst, filt, pad_mode: int = PAD_MEAN3, stream=None):
 
 def filter_views(src, dst, cos_table, filt, pad_mode: int = PAD_MEAN4, transpose: bool = True,
                  dtype: int = FDK_F32, stream=None, nviews: int | None = None,
-                 src_view_offset: int = 0, dst_view_offset: int = 0):
+                 src_view_offset: int = 0, dst_view_offset: int = 0, src_view_step: int = 1):
     """Fused cos weight + row filter.  src (views, nv, nu) float32 device tensor;
     dst (views, nu, nv) when transposed else (views, nv, nu), of the stored dtype.
-    Returns per-view (scale, offset) for FP16/BF16, else None."""
+    ``src_view_step`` reads every k-th source view (a sparse-view subset) without a
+    copy.  Returns per-view (scale, offset) for FP16/BF16, else None."""
     nv, nu = src.shape[-2], src.shape[-1]
-    n = src.shape[0] - src_view_offset if nviews is None else nviews
+    if nviews is None:
+        nviews = (src.shape[0] - src_view_offset + src_view_step - 1) // src_view_step
+    n = nviews
     view = nu * nv
     esize = 4 if dtype == FDK_F32 else 2
     so = np.zeros((max(n, 1), 2), np.float32) if dtype != FDK_F32 else None
     sp = C.c_void_p(src.data_ptr() + src_view_offset * view * 4)
     dp = C.c_void_p(dst.data_ptr() + dst_view_offset * view * esize)
-    _check(lib().fdk_filter_views(sp, view, dp, view, int(dtype), nu, nv, int(n),
+    _check(lib().fdk_filter_views(sp, view * int(src_view_step), dp, view, int(dtype), nu, nv, int(n),
                                   _dev_ptr(cos_table) if cos_table is not None else None,
                                   _dev_ptr(filt), int(filt.numel()), int(pad_mode),
                                   1 if transpose else 0,

@@ -1,0 +1,357 @@
+"""3-D residual U-Net for sparse-view FDK correction (no reference counterpart).
+
+Architecture: 3 levels, base 32 channels, two 3x3x3 conv + ReLU per level,
+2x2x2 average pooling down, nearest 2x upsampling up with channel-concatenated
+skips, a 1x1 output head, and a residual connection (output = input + net).
+
+Every operation runs in the library's HIP kernels (fdk_nn_*): BF16 MFMA
+convolutions with FP32 accumulation, BF16 channels-last activations, FP32
+master weights and Adam.  This module only sequences them; torch tensors serve
+as device buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _check, lib
+
+I16 = None  # torch.int16, set lazily (BF16 bit patterns)
+
+
+def _torch():
+    import torch
+    global I16
+    I16 = torch.int16
+    return torch
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _st():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _sig():
+    L = lib()
+    vp, i, i64, f = C.c_void_p, C.c_int, C.c_int64, C.c_float
+    L.fdk_nn_conv3d_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
+    L.fdk_nn_conv3d_wgrad.argtypes = [vp, vp, vp, i, i, i, i, i, i, vp]
+    L.fdk_nn_flip_weights.argtypes = [vp, vp, i, i, vp]
+    L.fdk_nn_bias_grad.argtypes = [vp, vp, i64, i, vp]
+    L.fdk_nn_relu_bwd.argtypes = [vp, vp, vp, i64, vp]
+    L.fdk_nn_avgpool2.argtypes = [vp, vp, i, i, i, i, i, i, vp]
+    L.fdk_nn_upsample2.argtypes = [vp, vp, i, i, i, i, i, i, i, i, vp]
+    L.fdk_nn_channel_copy.argtypes = [vp, i, i, vp, i, i, i, i64, vp]
+    L.fdk_nn_add.argtypes = [vp, vp, vp, i64, vp]
+    L.fdk_nn_head.argtypes = [vp, vp, vp, vp, vp, i64, i, vp]
+    L.fdk_nn_head_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i64, i, vp]
+    L.fdk_nn_mse.argtypes = [vp, vp, vp, vp, i64, vp]
+    L.fdk_nn_adam.argtypes = [vp, vp, vp, vp, vp, i64, f, f, f, f, i, i, i, vp]
+    L.fdk_nn_convert.argtypes = [vp, vp, i64, i, vp]
+    L.fdk_nn_box_copy.argtypes = [vp, vp, vp, vp, vp, vp, vp, f, vp]
+    L.fdk_poisson_noise.argtypes = [vp, i64, C.c_double, C.c_double, C.c_uint32, vp]
+    L.fdk_nn_mfma_probe.argtypes = [vp, vp, vp, vp]
+    return L
+
+
+_L = None
+
+
+def L():
+    global _L
+    if _L is None:
+        _L = _sig()
+    return _L
+
+
+# ----------------------------------------------------------- kernel wrappers
+def conv3d(x, w, bias, y, shape, cin, cout, relu=True, out_f32=False, resid=None):
+    n, d, h, wd = shape
+    _check(L().fdk_nn_conv3d_fwd(_p(x), _p(w), _p(bias), _p(resid), _p(y), n, d, h, wd, cin, cout,
+                                 int(relu), int(out_f32), _st()), "conv3d")
+    return y
+
+
+def conv3d_wgrad(x, dy, dw, shape, cin, cout):
+    n, d, h, wd = shape
+    _check(L().fdk_nn_conv3d_wgrad(_p(x), _p(dy), _p(dw), n, d, h, wd, cin, cout, _st()), "wgrad")
+
+
+def flip_weights(w, wb, cin, cout):
+    _check(L().fdk_nn_flip_weights(_p(w), _p(wb), cin, cout, _st()), "flip_weights")
+
+
+def bias_grad(dy, db, nvox, c):
+    _check(L().fdk_nn_bias_grad(_p(dy), _p(db), nvox, c, _st()), "bias_grad")
+
+
+def relu_bwd(dy, y, dz):
+    _check(L().fdk_nn_relu_bwd(_p(dy), _p(y), _p(dz), dy.numel(), _st()), "relu_bwd")
+    return dz
+
+
+def avgpool(x, y, shape, c, adjoint=False):
+    n, d, h, w = shape
+    _check(L().fdk_nn_avgpool2(_p(x), _p(y), n, d, h, w, c, int(adjoint), _st()), "avgpool")
+    return y
+
+
+def upsample(x, y, small_shape, cx, cy, c0, adjoint=False):
+    n, d, h, w = small_shape
+    _check(L().fdk_nn_upsample2(_p(x), _p(y), n, d, h, w, cx, cy, c0, int(adjoint), _st()),
+           "upsample")
+    return y
+
+
+def channel_copy(src, cs, s0, dst, cd, d0, cc, nvox):
+    _check(L().fdk_nn_channel_copy(_p(src), cs, s0, _p(dst), cd, d0, cc, nvox, _st()), "copy")
+
+
+def add(a, b, y):
+    _check(L().fdk_nn_add(_p(a), _p(b), _p(y), a.numel(), _st()), "add")
+    return y
+
+
+def to_bf16(src, dst):
+    _check(L().fdk_nn_convert(_p(src), _p(dst), src.numel(), 1, _st()), "to_bf16")
+    return dst
+
+
+def to_f32(src, dst):
+    _check(L().fdk_nn_convert(_p(src), _p(dst), src.numel(), 0, _st()), "to_f32")
+    return dst
+
+
+def mse(out, target, dout, loss_sum):
+    _check(L().fdk_nn_mse(_p(out), _p(target), _p(dout), _p(loss_sum), out.numel(), _st()), "mse")
+
+
+def box_copy(src, src_org, dst, dst_org, box, scale=1.0):
+    sd = (C.c_int * 3)(*src.shape[-3:])
+    so = (C.c_int * 3)(*src_org)
+    dd = (C.c_int * 3)(*dst.shape[-3:])
+    do = (C.c_int * 3)(*dst_org)
+    bx = (C.c_int * 3)(*box)
+    _check(L().fdk_nn_box_copy(_p(src), sd, so, _p(dst), dd, do, bx, float(scale), _st()),
+           "box_copy")
+
+
+def poisson_noise(proj, i0, mu, seed):
+    _check(L().fdk_poisson_noise(_p(proj), proj.numel(), float(i0), float(mu), int(seed) & 0xffffffff,
+                                 _st()), "poisson_noise")
+
+
+# -------------------------------------------------------------------- layers
+class Conv3:
+    """3x3x3 'same' convolution (+ bias, optional ReLU) with Adam state."""
+
+    def __init__(self, cin, cout, relu, rng, dev):
+        torch = _torch()
+        self.cin, self.cout, self.relu = cin, cout, relu
+        std = np.sqrt(2.0 / (27 * cin))
+        if cin == 1:
+            w = np.zeros((cout, 32), np.float32)
+            w[:, :27] = rng.standard_normal((cout, 27)) * std
+        else:
+            w = (rng.standard_normal((27, cout, cin)) * std).astype(np.float32)
+        self.w = torch.as_tensor(w).to(dev)
+        self.b = torch.zeros(cout, device=dev)
+        self.wbf = torch.empty(self.w.shape, dtype=I16, device=dev)
+        self.wflip = torch.empty((27, cin, cout), dtype=I16, device=dev) if cin > 1 else None
+        self.gw = torch.zeros_like(self.w)
+        self.gb = torch.zeros_like(self.b)
+        self.state = [torch.zeros_like(self.w), torch.zeros_like(self.w),
+                      torch.zeros_like(self.b), torch.zeros_like(self.b)]
+        self.refresh()
+
+    def refresh(self):
+        to_bf16(self.w, self.wbf)
+        if self.wflip is not None:
+            flip_weights(self.wbf, self.wflip, self.cin, self.cout)
+
+    def forward(self, x, shape, out_f32=False):
+        torch = _torch()
+        y = torch.empty(shape + (self.cout,), dtype=torch.float32 if out_f32 else I16,
+                        device=x.device)
+        return conv3d(x, self.wbf, self.b, y, shape, self.cin, self.cout, self.relu, out_f32)
+
+    def backward(self, dy, x, y, shape, need_dx=True):
+        """dy: gradient at the (post-ReLU) output y; x: the layer input."""
+        torch = _torch()
+        dz = relu_bwd(dy, y, torch.empty_like(dy)) if self.relu else dy
+        conv3d_wgrad(x, dz, self.gw, shape, self.cin, self.cout)
+        bias_grad(dz, self.gb, int(np.prod(shape)), self.cout)
+        if not need_dx:
+            return None
+        dx = torch.empty(shape + (self.cin,), dtype=I16, device=dy.device)
+        return conv3d(dz, self.wflip, None, dx, shape, self.cout, self.cin, relu=False)
+
+    def params(self):
+        return [(self.w, self.wbf, self.gw, self.state[0], self.state[1]),
+                (self.b, None, self.gb, self.state[2], self.state[3])]
+
+
+def adam(w, wbf, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
+    n = w.numel()
+    _check(L().fdk_nn_adam(_p(w), _p(wbf), _p(g), _p(m), _p(v), n, lr, b1, b2, eps, step, n, n,
+                           _st()), "adam")
+
+
+class UNet3D:
+    """Residual U-Net: out = x + head(dec1(...)).  Input / output (N, D, H, W)
+    FP32 tensors; D, H, W multiples of 16 and W a multiple of 128."""
+
+    def __init__(self, base: int = 32, seed: int = 0, device="cuda"):
+        torch = _torch()
+        rng = np.random.default_rng(seed)
+        c1, c2, c3 = base, 2 * base, 4 * base
+        self.c = (c1, c2, c3)
+        self.e1a = Conv3(1, c1, True, rng, device)
+        self.e1b = Conv3(c1, c1, True, rng, device)
+        self.e2a = Conv3(c1, c2, True, rng, device)
+        self.e2b = Conv3(c2, c2, True, rng, device)
+        self.ba = Conv3(c2, c3, True, rng, device)
+        self.bb = Conv3(c3, c3, True, rng, device)
+        self.d2a = Conv3(c3 + c2, c2, True, rng, device)
+        self.d2b = Conv3(c2, c2, True, rng, device)
+        self.d1a = Conv3(c2 + c1, c1, True, rng, device)
+        self.d1b = Conv3(c1, c1, True, rng, device)
+        self.hw = torch.as_tensor((rng.standard_normal(c1) * 1e-3).astype(np.float32)).to(device)
+        self.hb = torch.zeros(1, device=device)
+        self.ghw = torch.zeros_like(self.hw)
+        self.ghb = torch.zeros_like(self.hb)
+        self.hstate = [torch.zeros_like(self.hw), torch.zeros_like(self.hw),
+                       torch.zeros_like(self.hb), torch.zeros_like(self.hb)]
+        self.step_count = 0
+        self.cache = None
+
+    @property
+    def convs(self):
+        return [self.e1a, self.e1b, self.e2a, self.e2b, self.ba, self.bb, self.d2a, self.d2b,
+                self.d1a, self.d1b]
+
+    def forward(self, x32, keep=False):
+        torch = _torch()
+        c1, c2, c3 = self.c
+        N, D, H, W = x32.shape
+        s1_, s2_, s3_ = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
+        dev = x32.device
+        xb = to_bf16(x32, torch.empty(x32.shape, dtype=I16, device=dev))
+        e1a = self.e1a.forward(xb, s1_)
+        s1 = self.e1b.forward(e1a, s1_)
+        p1 = avgpool(s1, torch.empty(s2_ + (c1,), dtype=I16, device=dev), s1_, c1)
+        e2a = self.e2a.forward(p1, s2_)
+        s2 = self.e2b.forward(e2a, s2_)
+        p2 = avgpool(s2, torch.empty(s3_ + (c2,), dtype=I16, device=dev), s2_, c2)
+        ba = self.ba.forward(p2, s3_)
+        bb = self.bb.forward(ba, s3_)
+        cat2 = torch.empty(s2_ + (c3 + c2,), dtype=I16, device=dev)
+        upsample(bb, cat2, s3_, c3, c3 + c2, 0)
+        channel_copy(s2, c2, 0, cat2, c3 + c2, c3, c2, int(np.prod(s2_)))
+        d2a = self.d2a.forward(cat2, s2_)
+        d2 = self.d2b.forward(d2a, s2_)
+        cat1 = torch.empty(s1_ + (c2 + c1,), dtype=I16, device=dev)
+        upsample(d2, cat1, s2_, c2, c2 + c1, 0)
+        channel_copy(s1, c1, 0, cat1, c2 + c1, c2, c1, int(np.prod(s1_)))
+        d1a = self.d1a.forward(cat1, s1_)
+        d1 = self.d1b.forward(d1a, s1_)
+        out = torch.empty(x32.shape, dtype=torch.float32, device=dev)
+        _check(L().fdk_nn_head(_p(d1), _p(self.hw), _p(self.hb), _p(x32), _p(out),
+                               out.numel(), c1, _st()), "head")
+        if keep:
+            self.cache = dict(shapes=(s1_, s2_, s3_), xb=xb, e1a=e1a, s1=s1, p1=p1, e2a=e2a, s2=s2,
+                              p2=p2, ba=ba, bb=bb, cat2=cat2, d2a=d2a, d2=d2, cat1=cat1, d1a=d1a,
+                              d1=d1)
+        return out
+
+    def zero_grad(self):
+        for cv in self.convs:
+            cv.gw.zero_()
+            cv.gb.zero_()
+        self.ghw.zero_()
+        self.ghb.zero_()
+
+    def backward(self, dout):
+        torch = _torch()
+        c1, c2, c3 = self.c
+        k = self.cache
+        s1_, s2_, s3_ = k["shapes"]
+        dev = dout.device
+        n1, n2 = int(np.prod(s1_)), int(np.prod(s2_))
+        dd1 = torch.empty(s1_ + (c1,), dtype=I16, device=dev)
+        _check(L().fdk_nn_head_bwd(_p(dout), _p(k["d1"]), _p(self.hw), _p(dd1), _p(self.ghw),
+                                   _p(self.ghb), dout.numel(), c1, _st()), "head_bwd")
+        dd1a = self.d1b.backward(dd1, k["d1a"], k["d1"], s1_)
+        dcat1 = self.d1a.backward(dd1a, k["cat1"], k["d1a"], s1_)
+        dd2 = upsample(dcat1, torch.empty(s2_ + (c2,), dtype=I16, device=dev), s2_, c2, c2 + c1,
+                       0, adjoint=True)
+        ds1 = torch.empty(s1_ + (c1,), dtype=I16, device=dev)
+        channel_copy(dcat1, c2 + c1, c2, ds1, c1, 0, c1, n1)
+        dd2a = self.d2b.backward(dd2, k["d2a"], k["d2"], s2_)
+        dcat2 = self.d2a.backward(dd2a, k["cat2"], k["d2a"], s2_)
+        dbb = upsample(dcat2, torch.empty(s3_ + (c3,), dtype=I16, device=dev), s3_, c3, c3 + c2,
+                       0, adjoint=True)
+        ds2 = torch.empty(s2_ + (c2,), dtype=I16, device=dev)
+        channel_copy(dcat2, c3 + c2, c3, ds2, c2, 0, c2, n2)
+        dba = self.bb.backward(dbb, k["ba"], k["bb"], s3_)
+        dp2 = self.ba.backward(dba, k["p2"], k["ba"], s3_)
+        add(ds2, avgpool(dp2, torch.empty_like(ds2), s2_, c2, adjoint=True), ds2)
+        de2a = self.e2b.backward(ds2, k["e2a"], k["s2"], s2_)
+        dp1 = self.e2a.backward(de2a, k["p1"], k["e2a"], s2_)
+        add(ds1, avgpool(dp1, torch.empty_like(ds1), s1_, c1, adjoint=True), ds1)
+        de1a = self.e1b.backward(ds1, k["e1a"], k["s1"], s1_)
+        self.e1a.backward(de1a, k["xb"], k["e1a"], s1_, need_dx=False)
+
+    def adam_step(self, lr=1e-3):
+        self.step_count += 1
+        t = self.step_count
+        for cv in self.convs:
+            for (w, wbf, g, m, v) in cv.params():
+                adam(w, wbf, g, m, v, lr, t)
+            if cv.wflip is not None:
+                flip_weights(cv.wbf, cv.wflip, cv.cin, cv.cout)
+        adam(self.hw, None, self.ghw, self.hstate[0], self.hstate[1], lr, t)
+        adam(self.hb, None, self.ghb, self.hstate[2], self.hstate[3], lr, t)
+
+    def train_step(self, x32, target, lr=1e-3):
+        torch = _torch()
+        out = self.forward(x32, keep=True)
+        dout = torch.empty_like(out)
+        loss = torch.zeros(1, device=out.device)
+        mse(out, target, dout, loss)
+        self.zero_grad()
+        self.backward(dout)
+        self.adam_step(lr)
+        self.cache = None
+        return loss  # sum of squared errors (device)
+
+    # ------------------------------------------------------------ persistence
+    def state_dict(self):
+        d = {}
+        for i, cv in enumerate(self.convs):
+            d[f"conv{i}.w"] = cv.w.cpu().numpy()
+            d[f"conv{i}.b"] = cv.b.cpu().numpy()
+        d["head.w"] = self.hw.cpu().numpy()
+        d["head.b"] = self.hb.cpu().numpy()
+        return d
+
+    def load_state_dict(self, d):
+        torch = _torch()
+        for i, cv in enumerate(self.convs):
+            cv.w.copy_(torch.as_tensor(d[f"conv{i}.w"]))
+            cv.b.copy_(torch.as_tensor(d[f"conv{i}.b"]))
+            cv.refresh()
+        self.hw.copy_(torch.as_tensor(d["head.w"]))
+        self.hb.copy_(torch.as_tensor(d["head.b"]))
+
+    def save(self, path):
+        np.savez_compressed(path, **self.state_dict())
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.load_state_dict({k: z[k] for k in z.files})

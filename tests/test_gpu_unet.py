@@ -1,0 +1,249 @@
+"""U-Net kernels on the GPU against the FP64 oracle (oracle/nn_ref.py) and the
+whole network's gradients against torch float64 autograd on the CPU, all on
+the same BF16-rounded weights and inputs."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fdk_amd.unet as U
+from oracle import nn_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(torch, a):
+    return torch.as_tensor(R.to_bf16_bits(a).view(np.int16)).cuda()
+
+
+def from_bf(t):
+    return R.from_bf16_bits(t.cpu().numpy().view(np.uint16))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def test_mfma_16x16x32_bf16_layout(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(0)
+    A = R.bf16_round(rng.standard_normal((16, 32)))
+    B = R.bf16_round(rng.standard_normal((32, 16)))
+    a = torch.as_tensor(A.astype(np.float32)).cuda()
+    b = torch.as_tensor(B.astype(np.float32)).cuda()
+    d = torch.zeros((16, 16), device="cuda")
+    U.L().fdk_nn_mfma_probe(U._p(a), U._p(b), U._p(d), U._st())
+    torch.cuda.synchronize()
+    assert rel(d.cpu().numpy(), A @ B) < 1e-6
+
+
+CONV_CASES = [(32, 32, 64), (32, 64, 64), (64, 128, 32), (96, 32, 128), (192, 64, 64),
+              (128, 64, 32)]
+
+
+@pytest.mark.parametrize("cin,cout,W", CONV_CASES)
+def test_conv3d_forward(torch_cuda, cin, cout, W):
+    torch = torch_cuda
+    rng = np.random.default_rng(cin + cout)
+    shape = (2, 3, 8, W)
+    x = R.bf16_round(rng.standard_normal(shape + (cin,)))
+    w = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    y = torch.empty(shape + (cout,), dtype=torch.int16, device="cuda")
+    U.conv3d(bf(torch, x), bf(torch, w), torch.as_tensor(b).cuda(), y, shape, cin, cout, relu=True)
+    ref = np.maximum(R.conv3d(x, w, b), 0)
+    got = from_bf(y)
+    assert rel(got, ref) < 4e-3  # BF16 output rounding
+    # FP32 output with a residual: FP32 accumulation only
+    res = rng.standard_normal(shape + (cout,)).astype(np.float32)
+    y32 = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
+    U.conv3d(bf(torch, x), bf(torch, w), None, y32, shape, cin, cout, relu=False, out_f32=True,
+             resid=torch.as_tensor(res).cuda())
+    assert rel(y32.cpu().numpy(), R.conv3d(x, w) + res) < 1e-5
+
+
+def test_conv3d_forward_single_input_channel(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    shape = (1, 4, 8, 64)
+    x = R.bf16_round(rng.standard_normal(shape + (1,)))
+    w27 = R.bf16_round(rng.standard_normal((27, 32, 1)) / 5)
+    wpad = np.zeros((32, 32))
+    wpad[:, :27] = w27[:, :, 0].T
+    y = torch.empty(shape + (32,), dtype=torch.float32, device="cuda")
+    U.conv3d(bf(torch, x[..., 0]), bf(torch, wpad), None, y, shape, 1, 32, relu=False, out_f32=True)
+    assert rel(y.cpu().numpy(), R.conv3d(x, w27)) < 1e-5
+
+
+@pytest.mark.parametrize("cin,cout,W", [(32, 64, 64), (192, 64, 64), (96, 32, 128), (64, 128, 32)])
+def test_conv3d_input_and_weight_gradients(torch_cuda, cin, cout, W):
+    torch = torch_cuda
+    rng = np.random.default_rng(cin * 7 + cout)
+    shape = (2, 3, 8, W)
+    x = R.bf16_round(rng.standard_normal(shape + (cin,)))
+    dy = R.bf16_round(rng.standard_normal(shape + (cout,)))
+    w = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+    wf = torch.empty((27, cin, cout), dtype=torch.int16, device="cuda")
+    U.flip_weights(bf(torch, w), wf, cin, cout)
+    dx = torch.empty(shape + (cin,), dtype=torch.float32, device="cuda")
+    U.conv3d(bf(torch, dy), wf, None, dx, shape, cout, cin, relu=False, out_f32=True)
+    assert rel(dx.cpu().numpy(), R.conv3d_dx(dy, w)) < 1e-5
+    dw = torch.zeros((27, cout, cin), device="cuda")
+    U.conv3d_wgrad(bf(torch, x), bf(torch, dy), dw, shape, cin, cout)
+    U.conv3d_wgrad(bf(torch, x), bf(torch, dy), dw, shape, cin, cout)  # accumulates
+    assert rel(dw.cpu().numpy(), 2 * R.conv3d_dw(x, dy)) < 1e-5
+
+
+def test_conv3d_weight_gradient_single_input_channel(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(9)
+    shape = (1, 4, 8, 64)
+    x = R.bf16_round(rng.standard_normal(shape + (1,)))
+    dy = R.bf16_round(rng.standard_normal(shape + (32,)))
+    dw = torch.zeros((32, 32), device="cuda")
+    U.conv3d_wgrad(bf(torch, x[..., 0]), bf(torch, dy), dw, shape, 1, 32)
+    got = dw.cpu().numpy()
+    assert np.abs(got[:, 27:]).max() == 0.0
+    assert rel(got[:, :27], R.conv3d_dw(x, dy)[:, :, 0].T) < 1e-5
+
+
+def test_pool_upsample_copy_head_mse_adam(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(11)
+    full = (1, 4, 8, 16)
+    x = R.bf16_round(rng.standard_normal(full + (32,)))
+    p = torch.empty((1, 2, 4, 8, 32), dtype=torch.int16, device="cuda")
+    U.avgpool(bf(torch, x), p, full, 32)
+    assert rel(from_bf(p), R.avgpool2(x)) < 4e-3
+    s = R.bf16_round(rng.standard_normal((1, 2, 4, 8, 32)))
+    pa = torch.empty(full + (32,), dtype=torch.int16, device="cuda")
+    U.avgpool(bf(torch, s), pa, full, 32, adjoint=True)
+    assert rel(from_bf(pa), R.avgpool2_adjoint(s)) < 4e-3
+    cat = torch.zeros(full + (48,), dtype=torch.int16, device="cuda")
+    U.upsample(bf(torch, s), cat, (1, 2, 4, 8), 32, 48, 16)
+    assert rel(from_bf(cat)[..., 16:], R.upsample2(s)) == 0.0
+    up_adj = torch.empty((1, 2, 4, 8, 32), dtype=torch.int16, device="cuda")
+    U.upsample(cat, up_adj, (1, 2, 4, 8), 32, 48, 16, adjoint=True)
+    assert rel(from_bf(up_adj), R.upsample2_adjoint(from_bf(cat)[..., 16:])) < 4e-3
+    h = R.bf16_round(rng.standard_normal(full + (32,)))
+    hw = rng.standard_normal(32).astype(np.float32)
+    hb = np.array([0.25], np.float32)
+    xin = rng.standard_normal(full).astype(np.float32)
+    out = torch.empty(full, device="cuda")
+    # keep every argument tensor alive until the kernel has run
+    args = [bf(torch, h), torch.as_tensor(hw).cuda(), torch.as_tensor(hb).cuda(),
+            torch.as_tensor(xin).cuda()]
+    U.L().fdk_nn_head(*[U._p(a) for a in args], U._p(out), out.numel(), 32, U._st())
+    ref_out = xin + h @ hw + 0.25
+    assert rel(out.cpu().numpy(), ref_out) < 1e-6
+    tgt = rng.standard_normal(full).astype(np.float32)
+    dout = torch.empty(full, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    U.mse(out, torch.as_tensor(tgt).cuda(), dout, loss)
+    torch.cuda.synchronize()
+    assert loss.item() == pytest.approx(np.sum((ref_out - tgt) ** 2), rel=1e-4)
+    assert rel(dout.cpu().numpy(), 2 * (ref_out - tgt) / ref_out.size) < 1e-5
+    # Adam, two steps against a numpy restatement
+    w = rng.standard_normal(1000).astype(np.float32)
+    g1, g2 = rng.standard_normal((2, 1000)).astype(np.float32)
+    tw, tm, tv = (torch.as_tensor(w).cuda(), torch.zeros(1000, device="cuda"),
+                  torch.zeros(1000, device="cuda"))
+    wbf = torch.empty(1000, dtype=torch.int16, device="cuda")
+    mw, vw, ww = np.zeros(1000), np.zeros(1000), w.astype(np.float64)
+    for t, g in ((1, g1), (2, g2)):
+        U.adam(tw, wbf, torch.as_tensor(g).cuda(), tm, tv, 1e-3, t)
+        mw = 0.9 * mw + 0.1 * g
+        vw = 0.999 * vw + 0.001 * g.astype(np.float64) ** 2
+        ww = ww - 1e-3 * (mw / (1 - 0.9 ** t)) / (np.sqrt(vw / (1 - 0.999 ** t)) + 1e-8)
+    torch.cuda.synchronize()
+    assert np.abs(tw.cpu().numpy() - ww).max() < 1e-6
+    assert np.abs(from_bf(wbf) - R.bf16_round(tw.cpu().numpy())).max() == 0.0
+
+
+def _torch_unet(net, x):
+    """The same network in torch float64 on the CPU, weights taken from net."""
+    import torch
+    Fn = torch.nn.functional
+
+    def W(cv):
+        w = cv.w.cpu().numpy().astype(np.float64)
+        if cv.cin == 1:
+            w = w[:, :27].T[:, :, None]                       # (27, cout, 1)
+        w = R.bf16_round(w)
+        t = torch.as_tensor(w.transpose(1, 2, 0).reshape(cv.cout, cv.cin, 3, 3, 3))
+        return t.requires_grad_(True)
+
+    params = {}
+
+    def conv(cv, h):
+        w = W(cv)
+        b = torch.as_tensor(cv.b.cpu().numpy().astype(np.float64)).requires_grad_(True)
+        params[id(cv)] = (w, b)
+        return torch.relu(Fn.conv3d(h, w, b, padding=1))
+
+    xt = torch.as_tensor(R.bf16_round(x)[:, None]).requires_grad_(False)
+    e1 = conv(net.e1b, conv(net.e1a, xt))
+    e2 = conv(net.e2b, conv(net.e2a, Fn.avg_pool3d(e1, 2)))
+    bt = conv(net.bb, conv(net.ba, Fn.avg_pool3d(e2, 2)))
+    d2 = conv(net.d2b, conv(net.d2a, torch.cat([Fn.interpolate(bt, scale_factor=2), e2], 1)))
+    d1 = conv(net.d1b, conv(net.d1a, torch.cat([Fn.interpolate(d2, scale_factor=2), e1], 1)))
+    hw = torch.as_tensor(net.hw.cpu().numpy().astype(np.float64)).requires_grad_(True)
+    hb = torch.as_tensor(net.hb.cpu().numpy().astype(np.float64)).requires_grad_(True)
+    out = torch.as_tensor(x[:, None].astype(np.float64)) + \
+        torch.einsum("ncdhw,c->ndhw", d1, hw)[:, None] + hb
+    return out[:, 0], params, hw, hb
+
+
+def test_unet_forward_and_gradients_match_torch(torch_cuda):
+    torch = torch_cuda
+    net = U.UNet3D(base=32, seed=3)
+    with torch.no_grad():
+        net.hw.mul_(300.0)  # a head large enough that the net's output is not just x
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((1, 8, 16, 128)).astype(np.float32)
+    tgt = rng.standard_normal((1, 8, 16, 128)).astype(np.float32)
+    xd = torch.as_tensor(x).cuda()
+    out = net.forward(xd, keep=True)
+    dout = torch.empty_like(out)
+    loss = torch.zeros(1, device="cuda")
+    U.mse(out, torch.as_tensor(tgt).cuda(), dout, loss)
+    net.zero_grad()
+    net.backward(dout)
+    torch.cuda.synchronize()
+
+    ref_out, params, hw, hb = _torch_unet(net, x)
+    ref_loss = ((ref_out - torch.as_tensor(tgt.astype(np.float64))) ** 2).mean()
+    ref_loss.backward()
+    got = out.cpu().numpy() - x
+    want = ref_out.detach().numpy() - x
+    assert rel(got, want) < 3e-2  # BF16 activations between layers
+    cos = []
+    for cv in net.convs:
+        w, b = params[id(cv)]
+        gw_ref = w.grad.numpy().reshape(cv.cout, cv.cin, 27).transpose(2, 0, 1)
+        gw = cv.gw.cpu().numpy()
+        if cv.cin == 1:
+            gw = gw[:, :27].T[:, :, None]
+        c = float(np.sum(gw * gw_ref) / (np.linalg.norm(gw) * np.linalg.norm(gw_ref)))
+        cos.append(c)
+        assert c > 0.99, (cv.cin, cv.cout, c)
+        assert rel(gw, gw_ref) < 0.1
+        assert rel(cv.gb.cpu().numpy(), b.grad.numpy()) < 0.1
+    assert rel(net.ghw.cpu().numpy(), hw.grad.numpy()) < 3e-2
+    print("weight-gradient cosine similarity per conv:", [round(c, 5) for c in cos])
+
+
+def test_unet_training_reduces_loss(torch_cuda):
+    torch = torch_cuda
+    net = U.UNet3D(base=32, seed=5)
+    rng = np.random.default_rng(6)
+    # a smooth target and a noisy, blurred input: the net must learn a filter
+    z, y, x = np.meshgrid(np.linspace(-1, 1, 16), np.linspace(-1, 1, 16), np.linspace(-1, 1, 128),
+                          indexing="ij")
+    clean = (np.sin(3 * x) * np.cos(2 * y) + z)[None].astype(np.float32)
+    clean = np.concatenate([clean, -clean])
+    noisy = clean + 0.3 * rng.standard_normal(clean.shape).astype(np.float32)
+    xd, td = torch.as_tensor(noisy).cuda(), torch.as_tensor(clean).cuda()
+    losses = [net.train_step(xd, td, lr=1e-3).item() for _ in range(40)]
+    print("training loss:", [round(l, 1) for l in losses[::8]], "->", round(losses[-1], 1))
+    assert losses[-1] < 0.5 * losses[0]

@@ -1,0 +1,26 @@
+"""Host logic of the learned path (no GPU): tiling of a volume into
+overlapping 128^3 tiles and the scan geometry."""
+import pytest
+
+import fdk_amd.learned as Lr
+
+
+@pytest.mark.parametrize("n", [128, 200, 256, 512, 1024])
+def test_tiles_partition_the_axis(n):
+    t = Lr.tiles(n)
+    assert t[0][1] == 0 and t[-1][2] == n
+    for (s, lo, hi), (s2, lo2, hi2) in zip(t, t[1:]):
+        assert hi == lo2                      # written ranges partition [0, n)
+        assert s2 - s <= Lr.PATCH - Lr.OVERLAP
+    for s, lo, hi in t:
+        assert s <= lo < hi <= s + Lr.PATCH and 0 <= s and s + Lr.PATCH <= n
+    if n == 1024:
+        assert len(t) == 9                    # 9^3 = 729 tiles for the 1024^3 volume
+
+
+def test_c4_geometry():
+    p = Lr.geometry(1024, 1024)
+    assert (p.nx, p.nu, p.nproj) == (1024, 1024, 1024)
+    assert p.du == pytest.approx(0.4) and p.dx == pytest.approx(0.4 / 1.5)
+    q = Lr.geometry(256, 64)
+    assert q.nu * q.du == pytest.approx(409.6)
