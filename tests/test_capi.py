@@ -29,7 +29,10 @@ def test_library_exports_every_declared_symbol():
     assert len(names) > 30
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert set(names) == set(F.exported_symbols())
+    # every binding the Python layer declares is part of the header
+    assert set(F.exported_symbols()) <= set(names)
+    import fdk_amd.unet as U
+    U.L()  # binds the fdk_nn_* signatures without touching a device
 
 
 def test_device_count_without_gpu_is_not_an_error():
