@@ -109,6 +109,15 @@ _SIGS = {
                                         C.c_void_p]),
     "fdk_phantom_project": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                       C.c_void_p, C.c_void_p]),
+    "fdk_pipe_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p]),
+    "fdk_pipe_push_raw": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "fdk_pipe_push_filtered": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                         C.c_void_p]),
+    "fdk_pipe_finish": (C.c_int, [C.c_void_p]),
+    "fdk_pipe_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fdk_pipe_bp": (C.c_void_p, [C.c_void_p]),
+    "fdk_pipe_destroy": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -256,7 +265,20 @@ class BackProjectionFDK:
         self._h = h
         self.nx, self.ny, self.nz = nx, ny, nz
 
+    @classmethod
+    def _borrowed(cls, handle, nx, ny, nz, sub_top, sub_bottom, owner):
+        """A view of a handle that another object owns (the streaming pipeline)."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p(handle)
+        self._owner = owner
+        self.sub_top, self.sub_bottom = tuple(sub_top), tuple(sub_bottom)
+        self.nx, self.ny, self.nz = nx, ny, nz
+        return self
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self._h = None
+            return
         if getattr(self, "_h", None):
             lib().fdk_bp_destroy(self._h)
             self._h = None
@@ -334,6 +356,74 @@ class BackProjectionFDK:
         _check(lib().fdk_bp_get_volume_device(self._h, _dev_ptr(out), out.numel()),
                "get_volume_device")
         return out
+
+
+class Pipeline:
+    """Streaming reconstruction: host views in batches of up to 32 are copied
+    on one stream while the previous batch is filtered and backprojected on
+    another (fdk_pipe_*; the single-node form of cuFDK/mpiFDK.cpp:8-317).
+
+    ``push_raw`` takes raw views [count][nv][nu] FP32 (cos weighting and the
+    ramp filter run on the device); ``push_filtered`` takes filtered,
+    transposed views [count][nu][nv] in the store dtype (uint16/int16 bit
+    patterns for FP16/BF16, with per-view scale/offset).  Host arrays should be
+    page-locked (torch ``pin_memory=True``) for the copy to overlap."""
+
+    def __init__(self, para: Parameter, sub_top=None, sub_bottom=None, store_dtype: int = FDK_F32,
+                 pad_mode: int = PAD_MEAN4, slots: int = 2):
+        sub_top = (0, 0, 0, para.nx, para.ny, para.nz) if sub_top is None else sub_top
+        sub_bottom = (0, 0, 0, 0, 0, 0) if sub_bottom is None else sub_bottom
+        self._g = para.c()
+        t = (C.c_int64 * 6)(*[int(v) for v in sub_top])
+        b = (C.c_int64 * 6)(*[int(v) for v in sub_bottom])
+        h = C.c_void_p()
+        _check(lib().fdk_pipe_create(C.byref(self._g), t, b, int(store_dtype), int(pad_mode),
+                                     int(slots), C.byref(h)), "Pipeline")
+        self._h = h
+        self.store_dtype = store_dtype
+        self.bp = BackProjectionFDK._borrowed(lib().fdk_pipe_bp(h), para.nx, para.ny, para.nz,
+                                              sub_top, sub_bottom, self)
+
+    @staticmethod
+    def _host_ptr(a):
+        if hasattr(a, "data_ptr"):
+            if a.is_cuda:
+                raise ValueError("Pipeline takes host arrays")
+            if not a.is_contiguous():
+                raise ValueError("host views must be contiguous")
+            return C.c_void_p(a.data_ptr())
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("host views must be contiguous")
+        return C.c_void_p(a.ctypes.data)
+
+    def push_raw(self, views, count: int, first_view: int, view_step: int = 1):
+        _check(lib().fdk_pipe_push_raw(self._h, self._host_ptr(views), int(count),
+                                       int(first_view), int(view_step)), "push_raw")
+
+    def push_filtered(self, views, count: int, first_view: int, view_step: int = 1,
+                      scale_offset=None):
+        so = None if scale_offset is None else np.ascontiguousarray(scale_offset, np.float32)
+        _check(lib().fdk_pipe_push_filtered(self._h, self._host_ptr(views), int(count),
+                                            int(first_view), int(view_step),
+                                            _np_ptr(so) if so is not None else None),
+               "push_filtered")
+
+    def finish(self):
+        _check(lib().fdk_pipe_finish(self._h), "finish")
+
+    def stats(self) -> dict:
+        out = (C.c_double * 4)()
+        _check(lib().fdk_pipe_stats(self._h, out), "stats")
+        return {"views": int(out[0]), "h2d_bytes": out[1], "filter_ms": out[2], "bp_ms": out[3]}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if getattr(self, "bp", None) is not None:
+                self.bp._h = None
+            lib().fdk_pipe_destroy(self._h)
+            self._h = None
+
+    __del__ = close
 
 
 class BackProjectionRTK:

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
-SOURCES = ["capi.cpp", "geometry.cpp", "bp.hip", "filter.hip", "fp.hip", "nn.hip"]
+SOURCES = ["capi.cpp", "geometry.cpp", "bp.hip", "filter.hip", "fp.hip", "nn.hip", "pipeline.cpp"]
 
 
 def _run(cmd):

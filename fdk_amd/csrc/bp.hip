@@ -33,6 +33,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -300,6 +301,203 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
   }
 }
 
+// LDS-staged variant of bp_column_kernel (same blocks, columns, lanes and
+// results).  Per view, each block first agrees on the detector strip its four
+// columns need -- the rows iu..iu+1 of the columns whose u is inside the view
+// and the v range of the block's top (and mirror) z-runs -- and stages it into
+// LDS as FP32 with 16-byte (FP32) or 8-byte (FP16/BF16, dequantized on the way)
+// range-checked buffer loads; the taps are then LDS reads.  This trades the two
+// vector-memory gathers per voxel update for about a quarter of a 16-byte load.
+// R x CT (and R x CT for the mirror run) is sized on the host from the
+// matrices; every LDS index is clamped to the staged block, and a sample is
+// used only when both its row pair and its v are inside the view.
+template <int DT, bool SYM, int ZPT>
+__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale S, int R, int CT) {
+  extern __shared__ float stage[];  // [R][CT] top run, then [R][CT] mirror run
+  __shared__ int info[4][6];
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int tile = lb / (kTile * kTile);
+  const int within = lb % (kTile * kTile);
+  const int px = (tile % a.tiles_x) * kTile + within % kTile;
+  const int py = (tile / a.tiles_x) * kTile + within / kTile;
+  const int xs = 2 * px + (wave & 1);
+  const int ys = 2 * py + (wave >> 1);
+  const bool active = xs < a.nxs && ys < a.nys;  // inactive waves still join barriers
+  const int zb0 = blockIdx.y * (kWave * ZPT);
+  const int zb1 = min(zb0 + kWave * ZPT, a.zext) - 1;
+  const int zbase = zb0 + lane;
+
+  const float x = (float)(a.x0 + xs);
+  const float y = (float)(a.y0 + ys);
+  float acc[ZPT], accm[ZPT], zt[ZPT], zm[ZPT];
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    acc[j] = 0.f;
+    accm[j] = 0.f;
+    const int zl = zbase + j * kWave;
+    zt[j] = (float)(a.z0 + zl);
+    zm[j] = (float)(a.nz - 1 - (a.z0 + zl));
+  }
+  float* stm = stage + R * CT;
+  const int esize = DT == FDK_F32 ? 4 : 2;
+
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    const float wd = fmaf(m[8], x, fmaf(m[9], y, m[11]));
+    const float Z = 1.0f / wd;
+    const float U = fmaf(m[0], x, fmaf(m[1], y, m[3])) * Z;
+    const float Vb = fmaf(m[4], x, fmaf(m[5], y, m[7])) * Z;
+    const float dV = m[6] * Z;
+    const float Z2 = Z * Z;
+    const float fu0 = floorf(U);
+    const int iu = (int)fu0;
+    const bool uok = active && (unsigned)iu <= (unsigned)(a.H - 2);
+    const float fu = U - fu0;
+    const float w0 = (1.0f - fu) * Z2, w1 = fu * Z2;
+
+    if (lane == 0) {
+      const float va = fmaf((float)(a.z0 + zb0), dV, Vb), vb = fmaf((float)(a.z0 + zb1), dV, Vb);
+      info[wave][0] = uok ? iu : INT_MAX;
+      info[wave][1] = uok ? iu : INT_MIN;
+      info[wave][2] = uok ? (int)floorf(fminf(va, vb)) - 1 : INT_MAX;
+      info[wave][3] = uok ? (int)floorf(fmaxf(va, vb)) + 2 : INT_MIN;
+      if (SYM) {
+        const float ma = fmaf((float)(a.nz - 1 - (a.z0 + zb0)), dV, Vb);
+        const float mb = fmaf((float)(a.nz - 1 - (a.z0 + zb1)), dV, Vb);
+        info[wave][4] = uok ? (int)floorf(fminf(ma, mb)) - 1 : INT_MAX;
+        info[wave][5] = uok ? (int)floorf(fmaxf(ma, mb)) + 2 : INT_MIN;
+      }
+    }
+    __syncthreads();
+    int r0 = INT_MAX, r1 = INT_MIN, c0 = INT_MAX, c1 = INT_MIN, d0 = INT_MAX, d1 = INT_MIN;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      r0 = min(r0, info[w][0]);
+      r1 = max(r1, info[w][1]);
+      c0 = min(c0, info[w][2]);
+      c1 = max(c1, info[w][3]);
+      if (SYM) {
+        d0 = min(d0, info[w][4]);
+        d1 = max(d1, info[w][5]);
+      }
+    }
+    // Rows iu..iu+1 of every column inside the view, columns of the v runs
+    // (4-aligned start).  The block stages them when they fit in R x CT (the
+    // host sizes R and CT so that they do) and otherwise gathers this view
+    // straight from memory; either way every tap of a valid sample is read.
+    const int nrow = (r1 >= r0) ? r1 + 2 - r0 : 0;
+    c0 = max(c0, 0) & ~3;
+    c1 = min(c1, a.W - 1);
+    const int q4 = max((c1 - c0 + 4) / 4, 0);
+    int q4m = 0;
+    if (SYM) {
+      d0 = max(d0, 0) & ~3;
+      d1 = min(d1, a.W - 1);
+      q4m = max((d1 - d0 + 4) / 4, 0);
+    }
+    const bool fits = nrow <= R && 4 * q4 <= CT && 4 * q4m <= CT;
+    const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
+    if (fits) {
+      const __amdgpu_buffer_rsrc_t rv =
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+      const float sc = S.so[s][0], of = S.so[s][1];
+      const int ntop = nrow * q4, nall = ntop + (SYM ? nrow * q4m : 0);
+      for (int i = threadIdx.x; i < nall; i += 256) {
+        const bool mir = SYM && i >= ntop;
+        const int k = mir ? i - ntop : i;
+        const int qq = mir ? q4m : q4;
+        const int r = k / qq, q = k % qq;
+        const int col = (mir ? d0 : c0) + 4 * q;
+        const int off = ((r0 + r) * a.W + col) * esize;
+        float4 v;
+        if constexpr (DT == FDK_F32) {
+          v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rv, off, 0, 0));
+        } else {
+          const auto h = __builtin_amdgcn_raw_buffer_load_b64(rv, off, 0, 0);
+          float f[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const unsigned short bits = (unsigned short)(h[e >> 1] >> (16 * (e & 1)));
+            const float qv = DT == FDK_F16 ? __half2float(__ushort_as_half(bits))
+                                           : __uint_as_float((unsigned)bits << 16);
+            f[e] = fmaf(qv, sc, of);
+          }
+          v = make_float4(f[0], f[1], f[2], f[3]);
+        }
+        *reinterpret_cast<float4*>((mir ? stm : stage) + r * CT + 4 * q) = v;
+      }
+    }
+    __syncthreads();  // every wave has read info; the next view writes it after its compute
+    if (fits) {
+      const int lr = uok ? min(iu - r0, R - 2) : 0;
+      const float* rowt = stage + lr * CT;
+      const float* rowm = stm + lr * CT;
+#pragma unroll
+      for (int j = 0; j < ZPT; ++j) {
+        {
+          const float v = fmaf(zt[j], dV, Vb);
+          const float fv = floorf(v);
+          const int iv = (int)fv;
+          const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
+          const float t = v - fv;
+          const float* p = rowt + min(max(iv - c0, 0), CT - 2);
+          const float ra = fmaf(t, p[1] - p[0], p[0]);
+          const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
+          acc[j] += ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
+        }
+        if (SYM) {
+          const float v = fmaf(zm[j], dV, Vb);
+          const float fv = floorf(v);
+          const int iv = (int)fv;
+          const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
+          const float t = v - fv;
+          const float* p = rowm + min(max(iv - d0, 0), CT - 2);
+          const float ra = fmaf(t, p[1] - p[0], p[0]);
+          const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
+          accm[j] += ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
+        }
+      }
+    } else if (uok) {
+      const __amdgpu_buffer_rsrc_t g0 =
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t g1 = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+      const int rowbase = iu * a.W;
+      const float sc = S.so[s][0], off = S.so[s][1] * Z2;
+#pragma unroll
+      for (int j = 0; j < ZPT; ++j) {
+        if constexpr (DT == FDK_F32) {
+          acc[j] += sample_buf(g0, g1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0, w1);
+          if (SYM) accm[j] += sample_buf(g0, g1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0, w1);
+        } else {
+          bool ok;
+          const float c =
+              sample_buf16<DT>(g0, g1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0 * sc, w1 * sc, &ok);
+          acc[j] += ok ? c + off : 0.f;
+          if (SYM) {
+            const float cm =
+                sample_buf16<DT>(g0, g1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0 * sc, w1 * sc, &ok);
+            accm[j] += ok ? cm + off : 0.f;
+          }
+        }
+      }
+    }
+  }
+
+  if (!active) return;
+  float* col = a.vol + ((long long)xs * a.nys + ys) * (long long)a.nzz;
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    const int zl = zbase + j * kWave;
+    if (zl < a.zext) {
+      col[zl] += acc[j];
+      if (SYM) col[a.nzz - 1 - zl] += accm[j];
+    }
+  }
+}
+
 // General matrices (a z term in rows 0 or 2): every voxel does the full
 // projective map.  Same layout and sampling as bp_column_kernel, ZPT = 1.
 template <int DT, bool SYM>
@@ -455,8 +653,69 @@ int zpt_choice() {
   return z;
 }
 
+// Rows R and columns CT of the detector strip one block of bp_lds_kernel
+// stages per view, bounded from the matrices at the sub-volume's corners: the
+// u spread of a 2x2 column tile (one voxel step in x and in y), and the v run
+// of one wave (64 * ZPT voxels along z) plus the tile's v spread, with margins
+// for floor rounding and 4-column alignment.  false when a corner is behind
+// the source.
+bool stage_dims(const BpArgs& a, const BpMats& M, int zpt, int* R, int* CT) {
+  const double xs[2] = {(double)a.x0, (double)(a.x0 + a.nxs - 1)};
+  const double ys[2] = {(double)a.y0, (double)(a.y0 + a.nys - 1)};
+  const double zs[4] = {(double)a.z0, (double)(a.z0 + a.zext - 1), (double)(a.nz - 1 - a.z0),
+                        (double)(a.nz - 1 - (a.z0 + a.zext - 1))};
+  double su = 0.0, dvz = 0.0, sv = 0.0;
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    for (double x : xs)
+      for (double y : ys) {
+        const double w = m[8] * x + m[9] * y + m[11];
+        if (!(w > 0.0)) return false;
+        const double u = (m[0] * x + m[1] * y + m[3]) / w;
+        su = std::max(su, std::fabs((m[0] - u * m[8]) / w) + std::fabs((m[1] - u * m[9]) / w));
+        dvz = std::max(dvz, std::fabs(m[6] / w));
+        for (double z : zs) {
+          const double v = (m[4] * x + m[5] * y + m[6] * z + m[7]) / w;
+          sv = std::max(sv, std::fabs((m[4] - v * m[8]) / w) + std::fabs((m[5] - v * m[9]) / w));
+        }
+      }
+  }
+  *R = (int)std::ceil(1.25 * su) + 3;
+  const double span = dvz * (kWave * zpt - 1) + 1.25 * sv;
+  *CT = ((int)std::ceil(span) + 11) & ~3;
+  return true;
+}
+
+// 0: the LDS-staged kernel when it fits (default); FDK_BP_KERNEL=v2 selects
+// the direct-gather kernel.
+bool lds_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("FDK_BP_KERNEL");
+    return !(e && std::strcmp(e, "v2") == 0);
+  }();
+  return on;
+}
+
+template <int DT, int ZPT>
+bool launch_lds(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
+  if (!lds_enabled() || a.W % 4 != 0) return false;
+  if (DT != FDK_F32 && ((reinterpret_cast<uintptr_t>(a.proj) % 8) != 0 || a.view_stride % 4 != 0))
+    return false;
+  int R = 0, CT = 0;
+  if (!stage_dims(a, M, ZPT, &R, &CT)) return false;
+  const size_t lds = (size_t)R * CT * sizeof(float) * (h->sym ? 2 : 1);
+  if (lds > 64 * 1024) return false;
+  dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
+  if (h->sym)
+    hipLaunchKernelGGL((bp_lds_kernel<DT, true, ZPT>), grid, dim3(256), lds, h->stream, a, M, S, R, CT);
+  else
+    hipLaunchKernelGGL((bp_lds_kernel<DT, false, ZPT>), grid, dim3(256), lds, h->stream, a, M, S, R, CT);
+  return true;
+}
+
 template <int DT, int ZPT>
 void launch_column(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
+  if (launch_lds<DT, ZPT>(h, a, M, S, nblk)) return;
   dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
   if (h->sym) hipLaunchKernelGGL((bp_column_kernel<DT, true, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
   else hipLaunchKernelGGL((bp_column_kernel<DT, false, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);

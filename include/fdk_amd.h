@@ -25,6 +25,7 @@
  *                        and FilterEngine::Filtering (Filterlib/FilterLib.h:167-200)
  *   fdk_filter_proj_image FilterProjImage (host in, host out, in place)
  *                                                 FilterlibCufft/cudaFftFilter.cu:394
+ *   fdk_pipe_*           FilterThread / BpThread / PushBpData   cuFDK/mpiFDK.cpp:8-317
  *   fdk_fp_project       cudaCTForwardProjection_f32 / ForwardProjection
  *                                                 genProj/cudaIterType.cuh:280,
  *                                                 genProj/cudaForwardProjection.cu:338-487
@@ -201,6 +202,35 @@ int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nviews, const fl
  * image, transposed (nu rows of nv) when transpose is 1.  Synchronous. */
 int fdk_filter_frame_host(float* host_src, float* host_dst, int nu, int nv, const float* host_cos,
                           const float* host_filter, int N, int pad_mode, int transpose);
+
+/* ------------------------------------------------------------ streaming pipeline */
+/* Host views in batches of <= FDK_BP_MAX_PROJ -> copy stream -> (cos weighting
+ * + ramp filter) -> backprojection stream, with slots (2..4) device batch
+ * buffers in a ring: the single-node form of the reference's FilterThread /
+ * BpThread / SafeList pipeline and PushBpData batching
+ * (cuFDK/mpiFDK.cpp:8-317, 167-203).  The volume is a slab pair as in
+ * fdk_bp_create (nproj of g is the number of views of the scan; matrices come
+ * from g).  Projections are stored on the device in store_dtype. */
+typedef struct fdk_pipe* fdk_pipe_t;
+int fdk_pipe_create(const fdk_geometry* g, const int64_t box_top[6], const int64_t box_bottom[6],
+                    int store_dtype, int pad_mode, int slots, fdk_pipe_t* out);
+/* Raw views first_view + k*view_step (k < count), host [count][nv][nu] FP32:
+ * cos weighting, pad_mode row filter and transposition on the device, then
+ * backprojection.  Returns once the batch is copied (host_views may be reused). */
+int fdk_pipe_push_raw(fdk_pipe_t p, const float* host_views, int count, int first_view,
+                      int view_step);
+/* Views already filtered: host [count][nu][nv] in store_dtype, with 2*count
+ * floats {scale, offset} per view for FDK_F16 / FDK_BF16 (value = q*scale + offset). */
+int fdk_pipe_push_filtered(fdk_pipe_t p, const void* host_views, int count, int first_view,
+                           int view_step, const float* scale_offset);
+/* Waits for every pushed batch. */
+int fdk_pipe_finish(fdk_pipe_t p);
+/* out: views pushed, host-to-device bytes, filter ms, backprojection ms (device
+ * time of finished batches). */
+int fdk_pipe_stats(fdk_pipe_t p, double out[4]);
+/* The pipeline's backprojection handle (fdk_bp_get_volume, fdk_bp_volume_ptr, ...). */
+fdk_bp_t fdk_pipe_bp(fdk_pipe_t p);
+int fdk_pipe_destroy(fdk_pipe_t p);
 
 /* ------------------------------------------------------------ forward projection */
 /* Ray-marching projection of views [first, first+count) of geometry g through

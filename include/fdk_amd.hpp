@@ -9,6 +9,8 @@
 //   FrameT, FilterEngine        common/type.h:1893-2005, Filterlib/FilterLib.h:148-203
 //   Parameter                   common/type.h:4627-4895 (fields and the tables of init())
 //   cudaCTForwardProjection_f32 genProj/cudaIterType.cuh:280 (core arguments only)
+//   ProjectionImageT::LoadFolder/SaveToFolder, VolumeT::Save/Load
+//                               common/type.h:1796-1828, 3831-3858, 3256-3310
 //
 // Failures throw std::string, as the reference's VERIFY_TRUE / CUDA_CHECK_ERROR
 // do (common/type.h:194-196, common/cudaLib.cuh:49-56); the message is the
@@ -21,8 +23,11 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "fdk_amd.h"
 
@@ -34,6 +39,12 @@ inline std::string last_error() {
 }
 inline void check(int rc, const char* what) {
   if (rc < 0) throw std::string(what) + ": " + last_error();
+}
+// Creates every directory on the way to the last '/' of path (the reference's
+// MakePathDirIfNeed); existing directories are fine.
+inline void make_dirs(const std::string& path) {
+  for (size_t i = 1; i < path.size(); ++i)
+    if (path[i] == '/') ::mkdir(path.substr(0, i).c_str(), 0755);
 }
 }  // namespace fdk_amd_detail
 
@@ -268,10 +279,115 @@ struct ProjectionImageT {
     return true;
   }
   T* buffer() { return data.data(); }
+
+  // LoadFolder (type.h:3831-3844 with LoadProjectionImageFolder, 1796-1828):
+  // frame k*step <- <folder>img%04d.raw numbered k*step + start_index, each
+  // file exactly w*h samples of D, u fastest; the other frames stay zero.
+  // Every file's size is checked before any is read.
+  template <typename D = T>
+  bool LoadFolder(const std::string& folder, int w, int h, int c, int step = 1,
+                  int start_index = 0) {
+    if (w <= 0 || h <= 0 || c <= 0 || step <= 0) return false;
+    *this = ProjectionImageT(w, h, c);
+    const size_t n = (size_t)w * h;
+    std::vector<std::string> paths;
+    for (int i = 0; i < c; i += step) paths.push_back(frame_path(folder, i + start_index));
+    for (const std::string& p : paths) {
+      std::FILE* f = std::fopen(p.c_str(), "rb");
+      if (!f) return false;
+      std::fseek(f, 0, SEEK_END);
+      const long size = std::ftell(f);
+      std::fclose(f);
+      if (size < 0 || (size_t)size != n * sizeof(D)) return false;
+    }
+    std::vector<D> tmp(n);
+    for (size_t k = 0; k < paths.size(); ++k) {
+      std::FILE* f = std::fopen(paths[k].c_str(), "rb");
+      if (!f) return false;
+      const size_t got = std::fread(tmp.data(), sizeof(D), n, f);
+      std::fclose(f);
+      if (got != n) return false;
+      T* dst = data.data() + n * k * step;
+      for (size_t e = 0; e < n; ++e) dst[e] = (T)tmp[e];
+    }
+    return true;
+  }
+  // SaveToFolder (type.h:3845-3858): frames 0, step, 2*step, ... as
+  // <folder>img%04d.raw, raw samples of T, u fastest.
+  bool SaveToFolder(const std::string& folder, int step = 1) const {
+    fdk_amd_detail::make_dirs(folder);
+    const size_t n = (size_t)width * height;
+    for (int i = 0; i < count; i += step) {
+      std::FILE* f = std::fopen(frame_path(folder, i).c_str(), "wb");
+      if (!f) return false;
+      const size_t put = std::fwrite(data.data() + n * i, sizeof(T), n, f);
+      std::fclose(f);
+      if (put != n) return false;
+    }
+    return true;
+  }
+  static std::string frame_path(const std::string& folder, int index) {
+    char name[32];
+    std::snprintf(name, sizeof(name), "img%04d.raw", index);
+    return folder + name;
+  }
+
   int width, height, count;
   std::vector<T> data;
 };
 typedef ProjectionImageT<float> ProjectionImage;
+
+// A host volume nz x ny x nx (x fastest) with the reference's file format
+// (VolumeT::Save / Load, type.h:3256-3310): three int32 (nx, ny, nz), then
+// the samples slice by slice.
+template <typename T>
+struct VolumeT {
+  VolumeT() : nx(0), ny(0), nz(0) {}
+  VolumeT(int x, int y, int z) : nx(x), ny(y), nz(z), data((size_t)x * y * z) {}
+  T* buffer() { return data.data(); }
+  const T* buffer() const { return data.data(); }
+  bool Save(const std::string& path) const {
+    fdk_amd_detail::make_dirs(path);
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const int32_t hdr[3] = {nx, ny, nz};
+    bool ok = std::fwrite(hdr, sizeof(int32_t), 3, f) == 3;
+    ok = ok && std::fwrite(data.data(), sizeof(T), data.size(), f) == data.size();
+    std::fclose(f);
+    return ok;
+  }
+  bool Load(const std::string& path) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    int32_t hdr[3];
+    bool ok = std::fread(hdr, sizeof(int32_t), 3, f) == 3 && hdr[0] > 0 && hdr[1] > 0 &&
+              hdr[2] > 0;
+    if (ok) {
+      *this = VolumeT(hdr[0], hdr[1], hdr[2]);
+      ok = std::fread(data.data(), sizeof(T), data.size(), f) == data.size();
+    }
+    std::fclose(f);
+    return ok;
+  }
+  // One raw nx*ny file per slice, <folder>%06d.raw numbered z0 + k (the
+  // per-slice output of cuFDK/mpiFDK.cpp:548-575 without its MPI-grid suffix).
+  bool SaveSlices(const std::string& folder, int z0 = 0) const {
+    fdk_amd_detail::make_dirs(folder);
+    const size_t n = (size_t)nx * ny;
+    for (int k = 0; k < nz; ++k) {
+      char name[32];
+      std::snprintf(name, sizeof(name), "%06d.raw", z0 + k);
+      std::FILE* f = std::fopen((folder + name).c_str(), "wb");
+      if (!f) return false;
+      const size_t put = std::fwrite(data.data() + n * k, sizeof(T), n, f);
+      std::fclose(f);
+      if (put != n) return false;
+    }
+    return true;
+  }
+  int32_t nx, ny, nz;
+  std::vector<T> data;
+};
 
 // FilterProjImage: views filtered in place, DEPTH views per device batch;
 // returns the elapsed milliseconds as the reference does.
