@@ -113,7 +113,25 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
     dt = time.perf_counter() - t0
     nv_total = nv * reps
     gups = sub ** 3 * nv_total / dt / 1e9
-    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads,
+    extra = {}
+    if use_ref:
+        # the reference's 3dct row filter (FP64 FFT, filterImageRow) on whole
+        # views, one view per host thread, for the extrapolated full-scan time
+        from concurrent.futures import ThreadPoolExecutor
+        H = p.filter()
+        rng = np.random.default_rng(0)
+        views = [rng.standard_normal((p.nv, p.nu)).astype(np.float32) for _ in range(threads)]
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda v: O.ref_filter_rows_3dct(v, H), views))
+        per_view = (time.perf_counter() - t1) / len(views)
+        bp_s = float(p.nx) * p.ny * p.nz * p.nproj / (gups * 1e9)
+        extra = {"filter_s_per_view": round(per_view, 4),
+                 "extrapolated_full_scan_s": {"bp": round(bp_s, 1),
+                                              "filter": round(per_view * p.nproj, 1),
+                                              "note": "linear extrapolation in voxel updates and "
+                                                      "views from the measured samples"}}
+    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads, **extra,
             "kind": "reference" if use_ref else "port",
             "sample": (f"central {sub}^3 sub-volume of the {p.nx}^3 geometry x {nv} views "
                        f"x {reps} repeats ({sub ** 3 * nv_total:.3g} voxel-updates, {dt:.1f} s), "

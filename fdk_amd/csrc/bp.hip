@@ -304,19 +304,21 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
 // LDS-staged variant of bp_column_kernel (same blocks, columns, lanes and
 // results).  Per view, each block first agrees on the detector strip its four
 // columns need -- the rows iu..iu+1 of the columns whose u is inside the view
-// and the v range of the block's top (and mirror) z-runs -- and stages it into
-// LDS as FP32 with 16-byte (FP32) or 8-byte (FP16/BF16, dequantized on the way)
-// range-checked buffer loads; the taps are then LDS reads.  This trades the two
-// vector-memory gathers per voxel update for about a quarter of a 16-byte load.
-// R x CT (and R x CT for the mirror run) is sized on the host from the
-// matrices; every LDS index is clamped to the staged block, and a sample is
-// used only when both its row pair and its v are inside the view.
+// and the v range of each wave's whole top (and mirror) z-run -- and stages it
+// into LDS as FP32 with 16-byte (FP32) or 8-byte (FP16/BF16, dequantized on the
+// way) range-checked buffer loads, one wave per strip row; the taps are then
+// LDS reads.  This trades the two vector-memory gathers per voxel update for
+// about a quarter of a 16-byte load.  R x CT (and R x CT for the mirror run) is
+// sized on the host from the matrices with 4 spare columns; a view whose strip
+// does not fit is gathered straight from memory instead.  Samples outside the
+// view (and every sample of a column whose u is outside it) read a zero slot,
+// the last two columns of rows R-2 and R-1, which staging never writes.
 template <int DT, bool SYM, int ZPT>
 __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale S, int R, int CT) {
   extern __shared__ float stage[];  // [R][CT] top run, then [R][CT] mirror run
-  __shared__ int info[4][6];
+  __shared__ int4 info[6];          // info[k] = footprint value k of waves 0..3
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x % kWave;
   const int tile = lb / (kTile * kTile);
   const int within = lb % (kTile * kTile);
@@ -326,8 +328,14 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
   const int ys = 2 * py + (wave >> 1);
   const bool active = xs < a.nxs && ys < a.nys;  // inactive waves still join barriers
   const int zb0 = blockIdx.y * (kWave * ZPT);
-  const int zb1 = min(zb0 + kWave * ZPT, a.zext) - 1;
+  const int zb1 = zb0 + kWave * ZPT - 1;  // the whole run, also past zext
   const int zbase = zb0 + lane;
+  float* stm = stage + R * CT;
+  if (threadIdx.x < (SYM ? 8 : 4)) {  // the zero slot(s)
+    const int k = threadIdx.x;
+    float* base = (k >= 4 ? stm : stage) + (R - 2 + ((k >> 1) & 1)) * CT + CT - 2;
+    base[k & 1] = 0.0f;
+  }
 
   const float x = (float)(a.x0 + xs);
   const float y = (float)(a.y0 + ys);
@@ -340,8 +348,8 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
     zt[j] = (float)(a.z0 + zl);
     zm[j] = (float)(a.nz - 1 - (a.z0 + zl));
   }
-  float* stm = stage + R * CT;
   const int esize = DT == FDK_F32 ? 4 : 2;
+  int* inf = reinterpret_cast<int*>(info);
 
   for (int s = 0; s < a.nproj; ++s) {
     const float* m = M.m[s];
@@ -355,85 +363,95 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
     const int iu = (int)fu0;
     const bool uok = active && (unsigned)iu <= (unsigned)(a.H - 2);
     const float fu = U - fu0;
-    const float w0 = (1.0f - fu) * Z2, w1 = fu * Z2;
+    const float w0 = uok ? (1.0f - fu) * Z2 : 0.0f, w1 = uok ? fu * Z2 : 0.0f;
 
     if (lane == 0) {
       const float va = fmaf((float)(a.z0 + zb0), dV, Vb), vb = fmaf((float)(a.z0 + zb1), dV, Vb);
-      info[wave][0] = uok ? iu : INT_MAX;
-      info[wave][1] = uok ? iu : INT_MIN;
-      info[wave][2] = uok ? (int)floorf(fminf(va, vb)) - 1 : INT_MAX;
-      info[wave][3] = uok ? (int)floorf(fmaxf(va, vb)) + 2 : INT_MIN;
+      inf[0 * 4 + wave] = uok ? iu : INT_MAX;
+      inf[1 * 4 + wave] = uok ? iu : INT_MIN;
+      inf[2 * 4 + wave] = uok ? (int)floorf(fminf(va, vb)) - 1 : INT_MAX;
+      inf[3 * 4 + wave] = uok ? (int)floorf(fmaxf(va, vb)) + 2 : INT_MIN;
       if (SYM) {
         const float ma = fmaf((float)(a.nz - 1 - (a.z0 + zb0)), dV, Vb);
         const float mb = fmaf((float)(a.nz - 1 - (a.z0 + zb1)), dV, Vb);
-        info[wave][4] = uok ? (int)floorf(fminf(ma, mb)) - 1 : INT_MAX;
-        info[wave][5] = uok ? (int)floorf(fmaxf(ma, mb)) + 2 : INT_MIN;
+        inf[4 * 4 + wave] = uok ? (int)floorf(fminf(ma, mb)) - 1 : INT_MAX;
+        inf[5 * 4 + wave] = uok ? (int)floorf(fmaxf(ma, mb)) + 2 : INT_MIN;
       }
     }
     __syncthreads();
-    int r0 = INT_MAX, r1 = INT_MIN, c0 = INT_MAX, c1 = INT_MIN, d0 = INT_MAX, d1 = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      r0 = min(r0, info[w][0]);
-      r1 = max(r1, info[w][1]);
-      c0 = min(c0, info[w][2]);
-      c1 = max(c1, info[w][3]);
-      if (SYM) {
-        d0 = min(d0, info[w][4]);
-        d1 = max(d1, info[w][5]);
-      }
+    const int4 f0 = info[0], f1 = info[1], f2 = info[2], f3 = info[3];
+    const int r0 = __builtin_amdgcn_readfirstlane(min(min(f0.x, f0.y), min(f0.z, f0.w)));
+    const int r1 = __builtin_amdgcn_readfirstlane(max(max(f1.x, f1.y), max(f1.z, f1.w)));
+    int c0 = __builtin_amdgcn_readfirstlane(min(min(f2.x, f2.y), min(f2.z, f2.w)));
+    int c1 = __builtin_amdgcn_readfirstlane(max(max(f3.x, f3.y), max(f3.z, f3.w)));
+    int d0 = 0, d1 = -1;
+    if (SYM) {
+      const int4 f4 = info[4], f5 = info[5];
+      d0 = __builtin_amdgcn_readfirstlane(min(min(f4.x, f4.y), min(f4.z, f4.w)));
+      d1 = __builtin_amdgcn_readfirstlane(max(max(f5.x, f5.y), max(f5.z, f5.w)));
     }
-    // Rows iu..iu+1 of every column inside the view, columns of the v runs
-    // (4-aligned start).  The block stages them when they fit in R x CT (the
-    // host sizes R and CT so that they do) and otherwise gathers this view
-    // straight from memory; either way every tap of a valid sample is read.
+    // Rows iu..iu+1 of every column inside the view; columns of the v runs
+    // from a 4-aligned start.
     const int nrow = (r1 >= r0) ? r1 + 2 - r0 : 0;
+    if (nrow == 0) {  // no column of the block is inside the view
+      c0 = 0;
+      c1 = -1;
+      d0 = 0;
+      d1 = -1;
+    }
     c0 = max(c0, 0) & ~3;
     c1 = min(c1, a.W - 1);
-    const int q4 = max((c1 - c0 + 4) / 4, 0);
+    const int q4 = max((c1 - c0 + 4) >> 2, 0);
     int q4m = 0;
     if (SYM) {
       d0 = max(d0, 0) & ~3;
       d1 = min(d1, a.W - 1);
-      q4m = max((d1 - d0 + 4) / 4, 0);
+      q4m = max((d1 - d0 + 4) >> 2, 0);
     }
-    const bool fits = nrow <= R && 4 * q4 <= CT && 4 * q4m <= CT;
+    const bool fits = nrow <= R && 4 * q4 <= CT - 4 && 4 * q4m <= CT - 4;
     const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
     if (fits) {
       const __amdgpu_buffer_rsrc_t rv =
           __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
       const float sc = S.so[s][0], of = S.so[s][1];
-      const int ntop = nrow * q4, nall = ntop + (SYM ? nrow * q4m : 0);
-      for (int i = threadIdx.x; i < nall; i += 256) {
-        const bool mir = SYM && i >= ntop;
-        const int k = mir ? i - ntop : i;
+      const int nr_all = SYM ? 2 * nrow : nrow;
+      for (int rr = wave; rr < nr_all; rr += 4) {
+        const bool mir = SYM && rr >= nrow;
+        const int r = mir ? rr - nrow : rr;
         const int qq = mir ? q4m : q4;
-        const int r = k / qq, q = k % qq;
-        const int col = (mir ? d0 : c0) + 4 * q;
-        const int off = ((r0 + r) * a.W + col) * esize;
-        float4 v;
-        if constexpr (DT == FDK_F32) {
-          v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rv, off, 0, 0));
-        } else {
-          const auto h = __builtin_amdgcn_raw_buffer_load_b64(rv, off, 0, 0);
-          float f[4];
+        const int base = ((r0 + r) * a.W + (mir ? d0 : c0)) * esize;
+        float* dst = (mir ? stm : stage) + r * CT;
+        for (int q = lane; q < qq; q += kWave) {
+          float4 v;
+          if constexpr (DT == FDK_F32) {
+            v = __builtin_bit_cast(float4,
+                                   __builtin_amdgcn_raw_buffer_load_b128(rv, base + 16 * q, 0, 0));
+          } else {
+            const auto h = __builtin_amdgcn_raw_buffer_load_b64(rv, base + 8 * q, 0, 0);
+            float f[4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const unsigned short bits = (unsigned short)(h[e >> 1] >> (16 * (e & 1)));
-            const float qv = DT == FDK_F16 ? __half2float(__ushort_as_half(bits))
-                                           : __uint_as_float((unsigned)bits << 16);
-            f[e] = fmaf(qv, sc, of);
+            for (int e = 0; e < 4; ++e) {
+              const unsigned short bits = (unsigned short)(h[e >> 1] >> (16 * (e & 1)));
+              const float qv = DT == FDK_F16 ? __half2float(__ushort_as_half(bits))
+                                             : __uint_as_float((unsigned)bits << 16);
+              f[e] = fmaf(qv, sc, of);
+            }
+            v = make_float4(f[0], f[1], f[2], f[3]);
           }
-          v = make_float4(f[0], f[1], f[2], f[3]);
+          *reinterpret_cast<float4*>(dst + 4 * q) = v;
         }
-        *reinterpret_cast<float4*>((mir ? stm : stage) + r * CT + 4 * q) = v;
       }
     }
     __syncthreads();  // every wave has read info; the next view writes it after its compute
     if (fits) {
-      const int lr = uok ? min(iu - r0, R - 2) : 0;
-      const float* rowt = stage + lr * CT;
-      const float* rowm = stm + lr * CT;
+      // Row pair lr of the strip; a tap index iv reads rowt[iv - c0] (a
+      // sample inside the view always lies in the staged columns), any other
+      // sample the zero slot.
+      const int lr = uok ? iu - r0 : 0;
+      const int zs = (R - 2 - lr) * CT + CT - 2;  // zero slot, relative to row lr
+      const float* rowt = stage + lr * CT - c0;
+      const float* rowm = stm + lr * CT - d0;
+      const int zt_idx = zs + c0, zm_idx = zs + d0;
 #pragma unroll
       for (int j = 0; j < ZPT; ++j) {
         {
@@ -442,10 +460,10 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
           const int iv = (int)fv;
           const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
           const float t = v - fv;
-          const float* p = rowt + min(max(iv - c0, 0), CT - 2);
+          const float* p = rowt + (ok ? iv : zt_idx);
           const float ra = fmaf(t, p[1] - p[0], p[0]);
           const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
-          acc[j] += ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
+          acc[j] += fmaf(w0, ra, w1 * rb);
         }
         if (SYM) {
           const float v = fmaf(zm[j], dV, Vb);
@@ -453,10 +471,10 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
           const int iv = (int)fv;
           const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
           const float t = v - fv;
-          const float* p = rowm + min(max(iv - d0, 0), CT - 2);
+          const float* p = rowm + (ok ? iv : zm_idx);
           const float ra = fmaf(t, p[1] - p[0], p[0]);
           const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
-          accm[j] += ok ? fmaf(w0, ra, w1 * rb) : 0.0f;
+          accm[j] += fmaf(w0, ra, w1 * rb);
         }
       }
     } else if (uok) {
@@ -643,14 +661,18 @@ int fill_args(fdk_bp* h, int W, int H, long long view_stride, int esize, int npr
   return FDK_OK;
 }
 
-// Voxels per lane along z (tuning knob FDK_BP_ZPT = 4, 8 or 16; default 8).
-int zpt_choice() {
-  static const int z = [] {
+// Voxels per lane along z: 8, or 4 when that pads the column's z extent to
+// fewer wave runs (a 256-deep slab fills a ZPT 4 wave exactly and half of a
+// ZPT 8 one).  FDK_BP_ZPT = 4, 8 or 16 forces a value.
+int zpt_choice(int zext) {
+  static const int forced = [] {
     const char* e = std::getenv("FDK_BP_ZPT");
-    const int v = e ? std::atoi(e) : 8;
-    return (v == 4 || v == 8 || v == 16) ? v : 8;
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 16) ? v : 0;
   }();
-  return z;
+  if (forced) return forced;
+  const long long pad8 = (zext + 511LL) / 512 * 512, pad4 = (zext + 255LL) / 256 * 256;
+  return pad4 < pad8 ? 4 : 8;
 }
 
 // Rows R and columns CT of the detector strip one block of bp_lds_kernel
@@ -682,7 +704,7 @@ bool stage_dims(const BpArgs& a, const BpMats& M, int zpt, int* R, int* CT) {
   }
   *R = (int)std::ceil(1.25 * su) + 3;
   const double span = dvz * (kWave * zpt - 1) + 1.25 * sv;
-  *CT = ((int)std::ceil(span) + 11) & ~3;
+  *CT = (((int)std::ceil(span) + 11) & ~3) + 4;  // + the zero slot's spare columns
   return true;
 }
 
@@ -730,7 +752,7 @@ int launch_bp_dt(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, 
     if (h->sym) hipLaunchKernelGGL((bp_general_kernel<DT, true>), grid, dim3(256), 0, h->stream, a, M, S);
     else hipLaunchKernelGGL((bp_general_kernel<DT, false>), grid, dim3(256), 0, h->stream, a, M, S);
   } else {
-    switch (zpt_choice()) {
+    switch (zpt_choice(a.zext)) {
       case 4: launch_column<DT, 4>(h, a, M, S, nblk); break;
       case 16: launch_column<DT, 16>(h, a, M, S, nblk); break;
       default: launch_column<DT, 8>(h, a, M, S, nblk); break;
