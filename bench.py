@@ -49,6 +49,8 @@ def parse():
                     help="distinct host views, cycled through the scan")
     ap.add_argument("--stream-source", choices=["filtered", "raw"], default="filtered",
                     help="raw: FP32 raw views are streamed and filtered on the GPU")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned sparse-view comparison (1024^3, 256 vs 1024 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (3 times), the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -138,6 +140,27 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
                        "prefiltered FP32 "
                        "views, " + ("3dct BackProjection<float> compiled from the reference "
                                     "sources" if use_ref else "oracle port (FP64)"))}
+
+
+def learned_c4():
+    """The learned sparse-view comparison on a held-out phantom: FDK from all
+    1024 noisy views of a 1024^3 scan against FDK from every 4th view followed
+    by the trained U-Net (fdk_amd/weights/unet_c4.npz), with GPU wall times
+    (projections already in HBM) and RMSE / SSIM against the 1024-view FDK and
+    against the phantom."""
+    from fdk_amd import learned as Lr
+    from fdk_amd.unet import UNet3D
+    w = os.path.join(HERE, "fdk_amd", "weights", "unet_c4.npz")
+    if not os.path.exists(w):
+        return None
+    p = Lr.geometry(1024, 1024)
+    sc = Lr.Scanner(p)
+    net = UNet3D(base=32, seed=0)
+    net.load(w)
+    r = Lr.compare(net, sc, 100000)
+    r["weights"] = "fdk_amd/weights/unet_c4.npz"
+    r["phantom_seed"] = 100000
+    return r
 
 
 def stream_main(args):
@@ -367,6 +390,18 @@ def main():
             log(f"cpu baseline failed: {e}")
             cpu = None
 
+    learned = None
+    if not args.no_learned:
+        log("learned sparse-view comparison (1024^3)")
+        bp.close()
+        del store, raw, vol
+        torch.cuda.empty_cache()
+        try:
+            learned = learned_c4()
+        except Exception as e:  # reported, never decides the backprojection result
+            log(f"learned comparison failed: {e}")
+            learned = {"error": str(e)}
+
     out = {
         "metric": METRIC,
         "value": round(gups, 2),
@@ -391,6 +426,7 @@ def main():
                      "bytes_per_update": b_upd, "kernel": kernel,
                      "kernel_ms_avg": round(avg_ms, 3)},
         "cpu_baseline": cpu,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
     bp.close()

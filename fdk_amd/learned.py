@@ -137,24 +137,32 @@ def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4,
 
 
 def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
-          log=None, max_seconds: float | None = None):
-    """Adam on random batches of patch pairs (stops early after max_seconds).
-    Returns the per-step mean losses."""
+          log=None, max_seconds: float | None = None, lr_final: float = 0.05):
+    """Adam on random batches of patch pairs (stops early after max_seconds),
+    the learning rate following a cosine from lr to lr * lr_final over the run
+    (its progress is the larger of the step and time fractions).  Returns the
+    per-step mean losses."""
+    import math
     import torch
     rng = np.random.default_rng(seed)
     losses = []
     t0 = time.perf_counter()
+    progress_t = 0.0
     for s in range(steps):
         if max_seconds is not None and s % 10 == 0 and s > 0:
             torch.cuda.synchronize()
-            if time.perf_counter() - t0 > max_seconds:
+            el = time.perf_counter() - t0
+            progress_t = el / max_seconds
+            if el > max_seconds:
                 if log:
                     log(f"time budget reached after {s} steps")
                 break
+        progress = min(max(s / max(steps - 1, 1), progress_t), 1.0)
+        lr_s = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * progress)))
         idx = torch.as_tensor(rng.choice(xs.shape[0], size=batch, replace=False)).cuda()
         x = xs.index_select(0, idx)
         y = ys.index_select(0, idx)
-        loss = net.train_step(x, y, lr)
+        loss = net.train_step(x, y, lr_s)
         losses.append(loss)
         if log and (s % 25 == 0 or s == steps - 1):
             log(f"step {s}: mse {loss.item() / x.numel():.4e} ({time.perf_counter() - t0:.1f} s)")

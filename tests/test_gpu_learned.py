@@ -1,0 +1,40 @@
+"""The trained sparse-view network (fdk_amd/weights/unet_c4.npz, trained by
+``python -m fdk_amd.train_c4`` on seeds 0..159) on a held-out phantom: FDK from
+256 of 1024 noisy views plus the network must come closer to the 1024-view FDK
+than the 256-view FDK alone, on a central 256^3 box of the 1024^3 scan."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fdk_amd",
+                       "weights", "unet_c4.npz")
+
+
+def test_trained_network_improves_sparse_fdk_on_held_out_phantom(torch_cuda):
+    torch = torch_cuda
+    from fdk_amd import learned as Lr
+    from fdk_amd.unet import UNet3D
+    p = Lr.geometry(1024, 1024)
+    sc = Lr.Scanner(p)
+    net = UNet3D(base=32, seed=0)
+    net.load(WEIGHTS)
+    seed = 100001  # not a training seed
+    print(f"learned: held-out phantom seed {seed}, 256^3 box of 1024^3", flush=True)
+    ell = sc.phantom(seed)
+    raw = sc.project(ell, noise_seed=seed + 7919)
+    box = (384, 384, 384, 256, 256, 256)
+    store = torch.empty((p.nproj, p.nu, p.nv), device="cuda")
+    full = sc.fdk(raw, 1, box, store)
+    sparse = sc.fdk(raw, 4, box, store)
+    learned = Lr.infer_tiled(net, sparse)
+    torch.cuda.synchronize()
+    r_sparse, r_learned = Lr.rmse(sparse, full), Lr.rmse(learned, full)
+    s_sparse, s_learned = Lr.ssim_slices(sparse, full), Lr.ssim_slices(learned, full)
+    print(f"  RMSE vs 1024-view FDK: sparse {r_sparse:.4f} -> learned {r_learned:.4f}; "
+          f"SSIM {s_sparse:.4f} -> {s_learned:.4f}", flush=True)
+    assert np.isfinite(r_learned)
+    assert r_learned < 0.75 * r_sparse
+    assert s_learned > s_sparse
