@@ -301,210 +301,328 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
   }
 }
 
-// LDS-staged variant of bp_column_kernel (same blocks, columns, lanes and
-// results).  Per view, each block first agrees on the detector strip its four
-// columns need -- the rows iu..iu+1 of the columns whose u is inside the view
-// and the v range of each wave's whole top (and mirror) z-run -- and stages it
-// into LDS as FP32 with 16-byte (FP32) or 8-byte (FP16/BF16, dequantized on the
-// way) range-checked buffer loads, one wave per strip row; the taps are then
-// LDS reads.  This trades the two vector-memory gathers per voxel update for
-// about a quarter of a 16-byte load.  R x CT (and R x CT for the mirror run) is
-// sized on the host from the matrices with 4 spare columns; a view whose strip
-// does not fit is gathered straight from memory instead.  Samples outside the
-// view (and every sample of a column whose u is outside it) read a zero slot,
-// the last two columns of rows R-2 and R-1, which staging never writes.
-template <int DT, bool SYM, int ZPT>
-__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale S, int R, int CT) {
-  extern __shared__ float stage[];  // [R][CT] top run, then [R][CT] mirror run
-  __shared__ int4 info[6];          // info[k] = footprint value k of waves 0..3
-  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int lane = threadIdx.x % kWave;
-  const int tile = lb / (kTile * kTile);
-  const int within = lb % (kTile * kTile);
-  const int px = (tile % a.tiles_x) * kTile + within % kTile;
-  const int py = (tile / a.tiles_x) * kTile + within / kTile;
-  const int xs = 2 * px + (wave & 1);
-  const int ys = 2 * py + (wave >> 1);
-  const bool active = xs < a.nxs && ys < a.nys;  // inactive waves still join barriers
-  const int zb0 = blockIdx.y * (kWave * ZPT);
-  const int zb1 = zb0 + kWave * ZPT - 1;  // the whole run, also past zext
-  const int zbase = zb0 + lane;
-  float* stm = stage + R * CT;
-  if (threadIdx.x < (SYM ? 8 : 4)) {  // the zero slot(s)
-    const int k = threadIdx.x;
-    float* base = (k >= 4 ? stm : stage) + (R - 2 + ((k >> 1) & 1)) * CT + CT - 2;
-    base[k & 1] = 0.0f;
-  }
+// ------------------------------------------------------------- LDS-staged kernels
+// Per view, a block of bp_lds_kernel / bp_lds_pf_kernel (same blocks, columns
+// and lanes as bp_column_kernel) first agrees on the detector strip its four
+// columns need -- rows iu..iu+1 of the columns whose u is inside the view, and
+// the v range of each wave's whole top (and mirror) z-run -- and stages it into
+// LDS as FP32, one wave per strip row, with 16-byte (FP32) or 8-byte
+// (FP16/BF16, dequantized on the way) range-checked buffer loads; the taps are
+// then LDS reads.  This trades the two vector-memory gathers per voxel update
+// for about a quarter of a 16-byte load.  R x CT (and R x CT for the mirror
+// run) is sized on the host from the matrices with 4 spare columns; a view
+// whose strip does not fit is gathered straight from memory instead.  Samples
+// outside the view (and every sample of a column whose u is outside it) read a
+// zero slot, the last two columns of rows R-2 and R-1, which staging never
+// writes.
 
-  const float x = (float)(a.x0 + xs);
-  const float y = (float)(a.y0 + ys);
-  float acc[ZPT], accm[ZPT], zt[ZPT], zm[ZPT];
+// One column's view: v(z) = Vb + z*dV, the u-direction weights times 1/w^2
+// (zero when u is outside the view) and the row iu.
+struct ColView {
+  float Vb, dV, w0, w1, Z2;
+  int iu;
+  bool uok;
+};
+
+__device__ __forceinline__ ColView col_view(const float* m, float x, float y, int H, bool active) {
+  ColView c;
+  const float wd = fmaf(m[8], x, fmaf(m[9], y, m[11]));
+  const float Z = 1.0f / wd;
+  const float U = fmaf(m[0], x, fmaf(m[1], y, m[3])) * Z;
+  c.Vb = fmaf(m[4], x, fmaf(m[5], y, m[7])) * Z;
+  c.dV = m[6] * Z;
+  c.Z2 = Z * Z;
+  const float fu0 = floorf(U);
+  c.iu = (int)fu0;
+  c.uok = active && (unsigned)c.iu <= (unsigned)(H - 2);
+  const float fu = U - fu0;
+  c.w0 = c.uok ? (1.0f - fu) * c.Z2 : 0.0f;
+  c.w1 = c.uok ? fu * c.Z2 : 0.0f;
+  return c;
+}
+
+// Lane 0 of each wave publishes its footprint: info[k] = value k of waves 0..3
+// (row min/max, top-run column min/max, mirror-run column min/max).
+template <bool SYM>
+__device__ __forceinline__ void footprint_put(int4* info, int wave, const ColView& c, float zt0,
+                                              float zt1, float zm0, float zm1) {
+  int* inf = reinterpret_cast<int*>(info);
+  const float va = fmaf(zt0, c.dV, c.Vb), vb = fmaf(zt1, c.dV, c.Vb);
+  inf[0 * 4 + wave] = c.uok ? c.iu : INT_MAX;
+  inf[1 * 4 + wave] = c.uok ? c.iu : INT_MIN;
+  inf[2 * 4 + wave] = c.uok ? (int)floorf(fminf(va, vb)) - 1 : INT_MAX;
+  inf[3 * 4 + wave] = c.uok ? (int)floorf(fmaxf(va, vb)) + 2 : INT_MIN;
+  if (SYM) {
+    const float ma = fmaf(zm0, c.dV, c.Vb), mb = fmaf(zm1, c.dV, c.Vb);
+    inf[4 * 4 + wave] = c.uok ? (int)floorf(fminf(ma, mb)) - 1 : INT_MAX;
+    inf[5 * 4 + wave] = c.uok ? (int)floorf(fmaxf(ma, mb)) + 2 : INT_MIN;
+  }
+}
+
+// The block's strip for one view (wave-uniform values).
+struct Strip {
+  int r0, nrow, c0, q4, d0, q4m;
+  bool fits;
+};
+
+__device__ __forceinline__ int min4(int4 v) { return min(min(v.x, v.y), min(v.z, v.w)); }
+__device__ __forceinline__ int max4(int4 v) { return max(max(v.x, v.y), max(v.z, v.w)); }
+
+template <bool SYM>
+__device__ __forceinline__ Strip strip_get(const int4* info, int W, int R, int CT) {
+  Strip st;
+  st.r0 = __builtin_amdgcn_readfirstlane(min4(info[0]));
+  const int r1 = __builtin_amdgcn_readfirstlane(max4(info[1]));
+  int c0 = __builtin_amdgcn_readfirstlane(min4(info[2]));
+  int c1 = __builtin_amdgcn_readfirstlane(max4(info[3]));
+  int d0 = 0, d1 = -1;
+  if (SYM) {
+    d0 = __builtin_amdgcn_readfirstlane(min4(info[4]));
+    d1 = __builtin_amdgcn_readfirstlane(max4(info[5]));
+  }
+  st.nrow = (r1 >= st.r0) ? r1 + 2 - st.r0 : 0;
+  if (st.nrow == 0) {  // no column of the block is inside the view
+    c0 = 0;
+    c1 = -1;
+    d0 = 0;
+    d1 = -1;
+  }
+  st.c0 = max(c0, 0) & ~3;
+  st.q4 = max((min(c1, W - 1) - st.c0 + 4) >> 2, 0);
+  st.d0 = 0;
+  st.q4m = 0;
+  if (SYM) {
+    st.d0 = max(d0, 0) & ~3;
+    st.q4m = max((min(d1, W - 1) - st.d0 + 4) >> 2, 0);
+  }
+  st.fits = st.nrow <= R && 4 * st.q4 <= CT - 4 && 4 * st.q4m <= CT - 4;
+  return st;
+}
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// Four consecutive stored samples: raw bits (FP32: float4; 16-bit: 8 bytes).
+template <int DT>
+struct Quad {
+  typedef float4 raw;
+  __device__ static __forceinline__ raw load(__amdgpu_buffer_rsrc_t rv, int byte_off) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rv, byte_off, 0, 0));
+  }
+  __device__ static __forceinline__ float4 cvt(raw v, float, float) { return v; }
+};
+
+template <int DT>
+struct Quad16 {
+  typedef u32x2 raw;
+  __device__ static __forceinline__ raw load(__amdgpu_buffer_rsrc_t rv, int byte_off) {
+    return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rv, byte_off, 0, 0));
+  }
+  __device__ static __forceinline__ float4 cvt(raw h, float sc, float of) {
+    float f[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const unsigned short bits = (unsigned short)(h[e >> 1] >> (16 * (e & 1)));
+      const float qv = DT == FDK_F16 ? __half2float(__ushort_as_half(bits))
+                                     : __uint_as_float((unsigned)bits << 16);
+      f[e] = fmaf(qv, sc, of);
+    }
+    return make_float4(f[0], f[1], f[2], f[3]);
+  }
+};
+template <>
+struct Quad<FDK_F16> : Quad16<FDK_F16> {};
+template <>
+struct Quad<FDK_BF16> : Quad16<FDK_BF16> {};
+
+// Strip row rr (top rows first, then mirror rows) of a view: its source byte
+// offset in the view, its LDS row and its quad count.
+template <bool SYM>
+__device__ __forceinline__ void strip_row(const Strip& st, int rr, int W, int esize, int CT,
+                                          float* stage, float* stm, int* src, float** dst,
+                                          int* qq) {
+  const bool mir = SYM && rr >= st.nrow;
+  const int r = mir ? rr - st.nrow : rr;
+  *qq = mir ? st.q4m : st.q4;
+  *src = ((st.r0 + r) * W + (mir ? st.d0 : st.c0)) * esize;
+  *dst = (mir ? stm : stage) + r * CT;
+}
+
+// Stages a view's strip directly (loads, then LDS stores).
+template <int DT, bool SYM>
+__device__ __forceinline__ void strip_stage(const Strip& st, __amdgpu_buffer_rsrc_t rv, int W,
+                                            int CT, float sc, float of, float* stage, float* stm,
+                                            int wave, int lane) {
+  const int esize = DT == FDK_F32 ? 4 : 2;
+  const int nr_all = SYM ? 2 * st.nrow : st.nrow;
+  for (int rr = wave; rr < nr_all; rr += 4) {
+    int src, qq;
+    float* dst;
+    strip_row<SYM>(st, rr, W, esize, CT, stage, stm, &src, &dst, &qq);
+    for (int q = lane; q < qq; q += kWave)
+      *reinterpret_cast<float4*>(dst + 4 * q) =
+          Quad<DT>::cvt(Quad<DT>::load(rv, src + 4 * esize * q), sc, of);
+  }
+}
+
+// Register-prefetched staging: at most kPfRows strip rows per wave and kPfQuads
+// quads per lane and row (the host checks R and CT against these).
+constexpr int kPfRows = 3;
+constexpr int kPfQuads = 3;
+
+template <int DT, bool SYM>
+__device__ __forceinline__ void strip_issue(typename Quad<DT>::raw (&pre)[kPfRows][kPfQuads],
+                                            const Strip& st, __amdgpu_buffer_rsrc_t rv, int W,
+                                            int CT, float* stage, float* stm, int wave, int lane) {
+  const int esize = DT == FDK_F32 ? 4 : 2;
+  const int nr_all = SYM ? 2 * st.nrow : st.nrow;
+#pragma unroll
+  for (int i = 0; i < kPfRows; ++i) {
+    const int rr = wave + 4 * i;
+    if (rr < nr_all) {
+      int src, qq;
+      float* dst;
+      strip_row<SYM>(st, rr, W, esize, CT, stage, stm, &src, &dst, &qq);
+#pragma unroll
+      for (int k = 0; k < kPfQuads; ++k) {
+        const int q = lane + kWave * k;
+        if (q < qq) pre[i][k] = Quad<DT>::load(rv, src + 4 * esize * q);
+      }
+    }
+  }
+}
+
+template <int DT, bool SYM>
+__device__ __forceinline__ void strip_commit(const typename Quad<DT>::raw (&pre)[kPfRows][kPfQuads],
+                                             const Strip& st, int W, int CT, float sc, float of,
+                                             float* stage, float* stm, int wave, int lane) {
+  const int esize = DT == FDK_F32 ? 4 : 2;
+  const int nr_all = SYM ? 2 * st.nrow : st.nrow;
+#pragma unroll
+  for (int i = 0; i < kPfRows; ++i) {
+    const int rr = wave + 4 * i;
+    if (rr < nr_all) {
+      int src, qq;
+      float* dst;
+      strip_row<SYM>(st, rr, W, esize, CT, stage, stm, &src, &dst, &qq);
+#pragma unroll
+      for (int k = 0; k < kPfQuads; ++k) {
+        const int q = lane + kWave * k;
+        if (q < qq) *reinterpret_cast<float4*>(dst + 4 * q) = Quad<DT>::cvt(pre[i][k], sc, of);
+      }
+    }
+  }
+}
+
+// The taps of one view from the staged strip: a tap index iv reads
+// row[iv - c0] (a sample inside the view always lies in the staged columns),
+// any other sample the zero slot.
+template <bool SYM, int ZPT>
+__device__ __forceinline__ void compute_lds(float (&acc)[ZPT], float (&accm)[ZPT],
+                                            const float (&zt)[ZPT], const float (&zm)[ZPT],
+                                            const ColView& c, const Strip& st, const float* stage,
+                                            const float* stm, int W, int R, int CT) {
+  const int lr = c.uok ? c.iu - st.r0 : 0;
+  const int zs = (R - 2 - lr) * CT + CT - 2;  // zero slot, relative to row lr
+  const float* rowt = stage + lr * CT - st.c0;
+  const float* rowm = stm + lr * CT - st.d0;
+  const int zt_idx = zs + st.c0, zm_idx = zs + st.d0;
 #pragma unroll
   for (int j = 0; j < ZPT; ++j) {
-    acc[j] = 0.f;
-    accm[j] = 0.f;
-    const int zl = zbase + j * kWave;
-    zt[j] = (float)(a.z0 + zl);
-    zm[j] = (float)(a.nz - 1 - (a.z0 + zl));
+    {
+      const float v = fmaf(zt[j], c.dV, c.Vb);
+      const float fv = floorf(v);
+      const int iv = (int)fv;
+      const bool ok = c.uok && (unsigned)iv <= (unsigned)(W - 2);
+      const float t = v - fv;
+      const float* p = rowt + (ok ? iv : zt_idx);
+      const float ra = fmaf(t, p[1] - p[0], p[0]);
+      const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
+      acc[j] += fmaf(c.w0, ra, c.w1 * rb);
+    }
+    if (SYM) {
+      const float v = fmaf(zm[j], c.dV, c.Vb);
+      const float fv = floorf(v);
+      const int iv = (int)fv;
+      const bool ok = c.uok && (unsigned)iv <= (unsigned)(W - 2);
+      const float t = v - fv;
+      const float* p = rowm + (ok ? iv : zm_idx);
+      const float ra = fmaf(t, p[1] - p[0], p[0]);
+      const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
+      accm[j] += fmaf(c.w0, ra, c.w1 * rb);
+    }
   }
-  const int esize = DT == FDK_F32 ? 4 : 2;
-  int* inf = reinterpret_cast<int*>(info);
+}
 
-  for (int s = 0; s < a.nproj; ++s) {
-    const float* m = M.m[s];
-    const float wd = fmaf(m[8], x, fmaf(m[9], y, m[11]));
-    const float Z = 1.0f / wd;
-    const float U = fmaf(m[0], x, fmaf(m[1], y, m[3])) * Z;
-    const float Vb = fmaf(m[4], x, fmaf(m[5], y, m[7])) * Z;
-    const float dV = m[6] * Z;
-    const float Z2 = Z * Z;
-    const float fu0 = floorf(U);
-    const int iu = (int)fu0;
-    const bool uok = active && (unsigned)iu <= (unsigned)(a.H - 2);
-    const float fu = U - fu0;
-    const float w0 = uok ? (1.0f - fu) * Z2 : 0.0f, w1 = uok ? fu * Z2 : 0.0f;
-
-    if (lane == 0) {
-      const float va = fmaf((float)(a.z0 + zb0), dV, Vb), vb = fmaf((float)(a.z0 + zb1), dV, Vb);
-      inf[0 * 4 + wave] = uok ? iu : INT_MAX;
-      inf[1 * 4 + wave] = uok ? iu : INT_MIN;
-      inf[2 * 4 + wave] = uok ? (int)floorf(fminf(va, vb)) - 1 : INT_MAX;
-      inf[3 * 4 + wave] = uok ? (int)floorf(fmaxf(va, vb)) + 2 : INT_MIN;
+// The taps of one view gathered straight from memory (a strip that does not fit).
+template <int DT, bool SYM, int ZPT>
+__device__ __forceinline__ void compute_direct(float (&acc)[ZPT], float (&accm)[ZPT],
+                                               const float (&zt)[ZPT], const float (&zm)[ZPT],
+                                               const ColView& c, const BpArgs& a,
+                                               const char* vbase, float sc, float of) {
+  const __amdgpu_buffer_rsrc_t g0 =
+      __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t g1 = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+  const int rowbase = c.iu * a.W;
+  const float off = of * c.Z2;
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    if constexpr (DT == FDK_F32) {
+      acc[j] += sample_buf(g0, g1, fmaf(zt[j], c.dV, c.Vb), rowbase, a.W, c.w0, c.w1);
+      if (SYM) accm[j] += sample_buf(g0, g1, fmaf(zm[j], c.dV, c.Vb), rowbase, a.W, c.w0, c.w1);
+    } else {
+      bool ok;
+      const float q = sample_buf16<DT>(g0, g1, fmaf(zt[j], c.dV, c.Vb), rowbase, a.W, c.w0 * sc,
+                                       c.w1 * sc, &ok);
+      acc[j] += ok ? q + off : 0.f;
       if (SYM) {
-        const float ma = fmaf((float)(a.nz - 1 - (a.z0 + zb0)), dV, Vb);
-        const float mb = fmaf((float)(a.nz - 1 - (a.z0 + zb1)), dV, Vb);
-        inf[4 * 4 + wave] = uok ? (int)floorf(fminf(ma, mb)) - 1 : INT_MAX;
-        inf[5 * 4 + wave] = uok ? (int)floorf(fmaxf(ma, mb)) + 2 : INT_MIN;
-      }
-    }
-    __syncthreads();
-    const int4 f0 = info[0], f1 = info[1], f2 = info[2], f3 = info[3];
-    const int r0 = __builtin_amdgcn_readfirstlane(min(min(f0.x, f0.y), min(f0.z, f0.w)));
-    const int r1 = __builtin_amdgcn_readfirstlane(max(max(f1.x, f1.y), max(f1.z, f1.w)));
-    int c0 = __builtin_amdgcn_readfirstlane(min(min(f2.x, f2.y), min(f2.z, f2.w)));
-    int c1 = __builtin_amdgcn_readfirstlane(max(max(f3.x, f3.y), max(f3.z, f3.w)));
-    int d0 = 0, d1 = -1;
-    if (SYM) {
-      const int4 f4 = info[4], f5 = info[5];
-      d0 = __builtin_amdgcn_readfirstlane(min(min(f4.x, f4.y), min(f4.z, f4.w)));
-      d1 = __builtin_amdgcn_readfirstlane(max(max(f5.x, f5.y), max(f5.z, f5.w)));
-    }
-    // Rows iu..iu+1 of every column inside the view; columns of the v runs
-    // from a 4-aligned start.
-    const int nrow = (r1 >= r0) ? r1 + 2 - r0 : 0;
-    if (nrow == 0) {  // no column of the block is inside the view
-      c0 = 0;
-      c1 = -1;
-      d0 = 0;
-      d1 = -1;
-    }
-    c0 = max(c0, 0) & ~3;
-    c1 = min(c1, a.W - 1);
-    const int q4 = max((c1 - c0 + 4) >> 2, 0);
-    int q4m = 0;
-    if (SYM) {
-      d0 = max(d0, 0) & ~3;
-      d1 = min(d1, a.W - 1);
-      q4m = max((d1 - d0 + 4) >> 2, 0);
-    }
-    const bool fits = nrow <= R && 4 * q4 <= CT - 4 && 4 * q4m <= CT - 4;
-    const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
-    if (fits) {
-      const __amdgpu_buffer_rsrc_t rv =
-          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
-      const float sc = S.so[s][0], of = S.so[s][1];
-      const int nr_all = SYM ? 2 * nrow : nrow;
-      for (int rr = wave; rr < nr_all; rr += 4) {
-        const bool mir = SYM && rr >= nrow;
-        const int r = mir ? rr - nrow : rr;
-        const int qq = mir ? q4m : q4;
-        const int base = ((r0 + r) * a.W + (mir ? d0 : c0)) * esize;
-        float* dst = (mir ? stm : stage) + r * CT;
-        for (int q = lane; q < qq; q += kWave) {
-          float4 v;
-          if constexpr (DT == FDK_F32) {
-            v = __builtin_bit_cast(float4,
-                                   __builtin_amdgcn_raw_buffer_load_b128(rv, base + 16 * q, 0, 0));
-          } else {
-            const auto h = __builtin_amdgcn_raw_buffer_load_b64(rv, base + 8 * q, 0, 0);
-            float f[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const unsigned short bits = (unsigned short)(h[e >> 1] >> (16 * (e & 1)));
-              const float qv = DT == FDK_F16 ? __half2float(__ushort_as_half(bits))
-                                             : __uint_as_float((unsigned)bits << 16);
-              f[e] = fmaf(qv, sc, of);
-            }
-            v = make_float4(f[0], f[1], f[2], f[3]);
-          }
-          *reinterpret_cast<float4*>(dst + 4 * q) = v;
-        }
-      }
-    }
-    __syncthreads();  // every wave has read info; the next view writes it after its compute
-    if (fits) {
-      // Row pair lr of the strip; a tap index iv reads rowt[iv - c0] (a
-      // sample inside the view always lies in the staged columns), any other
-      // sample the zero slot.
-      const int lr = uok ? iu - r0 : 0;
-      const int zs = (R - 2 - lr) * CT + CT - 2;  // zero slot, relative to row lr
-      const float* rowt = stage + lr * CT - c0;
-      const float* rowm = stm + lr * CT - d0;
-      const int zt_idx = zs + c0, zm_idx = zs + d0;
-#pragma unroll
-      for (int j = 0; j < ZPT; ++j) {
-        {
-          const float v = fmaf(zt[j], dV, Vb);
-          const float fv = floorf(v);
-          const int iv = (int)fv;
-          const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
-          const float t = v - fv;
-          const float* p = rowt + (ok ? iv : zt_idx);
-          const float ra = fmaf(t, p[1] - p[0], p[0]);
-          const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
-          acc[j] += fmaf(w0, ra, w1 * rb);
-        }
-        if (SYM) {
-          const float v = fmaf(zm[j], dV, Vb);
-          const float fv = floorf(v);
-          const int iv = (int)fv;
-          const bool ok = uok && (unsigned)iv <= (unsigned)(a.W - 2);
-          const float t = v - fv;
-          const float* p = rowm + (ok ? iv : zm_idx);
-          const float ra = fmaf(t, p[1] - p[0], p[0]);
-          const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
-          accm[j] += fmaf(w0, ra, w1 * rb);
-        }
-      }
-    } else if (uok) {
-      const __amdgpu_buffer_rsrc_t g0 =
-          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
-      const __amdgpu_buffer_rsrc_t g1 = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
-      const int rowbase = iu * a.W;
-      const float sc = S.so[s][0], off = S.so[s][1] * Z2;
-#pragma unroll
-      for (int j = 0; j < ZPT; ++j) {
-        if constexpr (DT == FDK_F32) {
-          acc[j] += sample_buf(g0, g1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0, w1);
-          if (SYM) accm[j] += sample_buf(g0, g1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0, w1);
-        } else {
-          bool ok;
-          const float c =
-              sample_buf16<DT>(g0, g1, fmaf(zt[j], dV, Vb), rowbase, a.W, w0 * sc, w1 * sc, &ok);
-          acc[j] += ok ? c + off : 0.f;
-          if (SYM) {
-            const float cm =
-                sample_buf16<DT>(g0, g1, fmaf(zm[j], dV, Vb), rowbase, a.W, w0 * sc, w1 * sc, &ok);
-            accm[j] += ok ? cm + off : 0.f;
-          }
-        }
+        const float qm = sample_buf16<DT>(g0, g1, fmaf(zm[j], c.dV, c.Vb), rowbase, a.W,
+                                          c.w0 * sc, c.w1 * sc, &ok);
+        accm[j] += ok ? qm + off : 0.f;
       }
     }
   }
+}
 
-  if (!active) return;
+// Shared prologue of the LDS kernels: block/column indices, z coordinates,
+// zero slot(s).
+#define FDK_LDS_PROLOGUE                                                              \
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);                              \
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);              \
+  const int lane = threadIdx.x % kWave;                                              \
+  const int tile = lb / (kTile * kTile);                                             \
+  const int within = lb % (kTile * kTile);                                           \
+  const int px = (tile % a.tiles_x) * kTile + within % kTile;                        \
+  const int py = (tile / a.tiles_x) * kTile + within / kTile;                        \
+  const int xs = 2 * px + (wave & 1);                                                \
+  const int ys = 2 * py + (wave >> 1);                                               \
+  const bool active = xs < a.nxs && ys < a.nys; /* inactive waves join barriers */   \
+  const int zb0 = blockIdx.y * (kWave * ZPT);                                        \
+  const int zb1 = zb0 + kWave * ZPT - 1; /* the whole run, also past zext */         \
+  const int zbase = zb0 + lane;                                                      \
+  float* stm = stage + R * CT;                                                       \
+  if (threadIdx.x < (SYM ? 8 : 4)) {                                                 \
+    const int k = threadIdx.x;                                                       \
+    float* zb = (k >= 4 ? stm : stage) + (R - 2 + ((k >> 1) & 1)) * CT + CT - 2;     \
+    zb[k & 1] = 0.0f;                                                                \
+  }                                                                                  \
+  const float x = (float)(a.x0 + xs);                                                \
+  const float y = (float)(a.y0 + ys);                                                \
+  float acc[ZPT], accm[ZPT], zt[ZPT], zm[ZPT];                                       \
+  _Pragma("unroll") for (int j = 0; j < ZPT; ++j) {                                  \
+    acc[j] = 0.f;                                                                    \
+    accm[j] = 0.f;                                                                   \
+    const int zl = zbase + j * kWave;                                                \
+    zt[j] = (float)(a.z0 + zl);                                                      \
+    zm[j] = (float)(a.nz - 1 - (a.z0 + zl));                                         \
+  }                                                                                  \
+  const float zt0 = (float)(a.z0 + zb0), zt1 = (float)(a.z0 + zb1);                  \
+  const float zm0 = (float)(a.nz - 1 - (a.z0 + zb0));                                \
+  const float zm1 = (float)(a.nz - 1 - (a.z0 + zb1));
+
+template <bool SYM, int ZPT>
+__device__ __forceinline__ void column_write(const BpArgs& a, int xs, int ys, int zbase,
+                                             const float (&acc)[ZPT], const float (&accm)[ZPT]) {
   float* col = a.vol + ((long long)xs * a.nys + ys) * (long long)a.nzz;
 #pragma unroll
   for (int j = 0; j < ZPT; ++j) {
@@ -514,6 +632,86 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
       if (SYM) col[a.nzz - 1 - zl] += accm[j];
     }
   }
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const BpArgs& a, int s) {
+  const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+}
+
+// Stage, then compute, every view (two block barriers per view).
+template <int DT, bool SYM, int ZPT>
+__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale S, int R, int CT) {
+  extern __shared__ float stage[];  // [R][CT] top run, then [R][CT] mirror run
+  __shared__ int4 info[6];
+  FDK_LDS_PROLOGUE
+  for (int s = 0; s < a.nproj; ++s) {
+    const ColView c = col_view(M.m[s], x, y, a.H, active);
+    if (lane == 0) footprint_put<SYM>(info, wave, c, zt0, zt1, zm0, zm1);
+    __syncthreads();
+    const Strip st = strip_get<SYM>(info, a.W, R, CT);
+    if (st.fits)
+      strip_stage<DT, SYM>(st, view_rsrc(a, s), a.W, CT, S.so[s][0], S.so[s][1], stage, stm, wave,
+                           lane);
+    __syncthreads();  // every wave has read info; the next view writes it after its compute
+    if (st.fits)
+      compute_lds<SYM, ZPT>(acc, accm, zt, zm, c, st, stage, stm, a.W, R, CT);
+    else if (c.uok)
+      compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, c, a,
+                                   static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
+                                   S.so[s][0], S.so[s][1]);
+  }
+  if (active) column_write<SYM, ZPT>(a, xs, ys, zbase, acc, accm);
+}
+
+// As bp_lds_kernel, with view s+1's strip loaded into registers while view s
+// is computed from LDS, and written to LDS after the block's last read of
+// view s: the staging loads' latency hides under the compute.  Needs at most
+// kPfRows strip rows per wave and kPfQuads quads per lane (checked on the host).
+template <int DT, bool SYM, int ZPT>
+__global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpScale S, int R,
+                                                        int CT) {
+  extern __shared__ float stage[];
+  __shared__ int4 info[2][6];  // footprints of alternate views
+  FDK_LDS_PROLOGUE
+  if (a.nproj <= 0) return;
+  {
+    const ColView c = col_view(M.m[0], x, y, a.H, active);
+    if (lane == 0) footprint_put<SYM>(info[0], wave, c, zt0, zt1, zm0, zm1);
+  }
+  __syncthreads();
+  Strip cur = strip_get<SYM>(info[0], a.W, R, CT);
+  if (cur.fits)
+    strip_stage<DT, SYM>(cur, view_rsrc(a, 0), a.W, CT, S.so[0][0], S.so[0][1], stage, stm, wave,
+                         lane);
+  for (int s = 0; s < a.nproj; ++s) {
+    const bool more = s + 1 < a.nproj;
+    if (more && lane == 0) {
+      const ColView cn = col_view(M.m[s + 1], x, y, a.H, active);
+      footprint_put<SYM>(info[(s + 1) & 1], wave, cn, zt0, zt1, zm0, zm1);
+    }
+    __syncthreads();  // strip s is in LDS; the footprint of view s+1 is visible
+    Strip nxt;
+    nxt.fits = false;
+    typename Quad<DT>::raw pre[kPfRows][kPfQuads];
+    if (more) {
+      nxt = strip_get<SYM>(info[(s + 1) & 1], a.W, R, CT);
+      if (nxt.fits) strip_issue<DT, SYM>(pre, nxt, view_rsrc(a, s + 1), a.W, CT, stage, stm, wave, lane);
+    }
+    const ColView c = col_view(M.m[s], x, y, a.H, active);
+    if (cur.fits)
+      compute_lds<SYM, ZPT>(acc, accm, zt, zm, c, cur, stage, stm, a.W, R, CT);
+    else if (c.uok)
+      compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, c, a,
+                                   static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
+                                   S.so[s][0], S.so[s][1]);
+    __syncthreads();  // every wave is done with strip s
+    if (nxt.fits)
+      strip_commit<DT, SYM>(pre, nxt, a.W, CT, S.so[s + 1][0], S.so[s + 1][1], stage, stm, wave,
+                            lane);
+    cur = nxt;
+  }
+  if (active) column_write<SYM, ZPT>(a, xs, ys, zbase, acc, accm);
 }
 
 // General matrices (a z term in rows 0 or 2): every voxel does the full
@@ -708,19 +906,22 @@ bool stage_dims(const BpArgs& a, const BpMats& M, int zpt, int* R, int* CT) {
   return true;
 }
 
-// 0: the LDS-staged kernel when it fits (default); FDK_BP_KERNEL=v2 selects
-// the direct-gather kernel.
-bool lds_enabled() {
-  static const bool on = [] {
+// Kernel choice: FDK_BP_KERNEL=v2 selects the direct-gather bp_column_kernel,
+// v3 the LDS-staged bp_lds_kernel; by default bp_lds_pf_kernel runs when its
+// register prefetch covers the strip, else bp_lds_kernel, else the direct one.
+int kernel_choice() {
+  static const int k = [] {
     const char* e = std::getenv("FDK_BP_KERNEL");
-    return !(e && std::strcmp(e, "v2") == 0);
+    if (e && std::strcmp(e, "v2") == 0) return 2;
+    if (e && std::strcmp(e, "v3") == 0) return 3;
+    return 4;
   }();
-  return on;
+  return k;
 }
 
 template <int DT, int ZPT>
 bool launch_lds(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
-  if (!lds_enabled() || a.W % 4 != 0) return false;
+  if (kernel_choice() < 3 || a.W % 4 != 0) return false;
   if (DT != FDK_F32 && ((reinterpret_cast<uintptr_t>(a.proj) % 8) != 0 || a.view_stride % 4 != 0))
     return false;
   int R = 0, CT = 0;
@@ -728,10 +929,20 @@ bool launch_lds(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, u
   const size_t lds = (size_t)R * CT * sizeof(float) * (h->sym ? 2 : 1);
   if (lds > 64 * 1024) return false;
   dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
-  if (h->sym)
+  const bool pf = DT == FDK_F32 && kernel_choice() >= 4 && (h->sym ? 2 * R : R) <= 4 * kPfRows &&
+                  CT <= 4 * kWave * kPfQuads;
+  if (pf) {
+    if (h->sym)
+      hipLaunchKernelGGL((bp_lds_pf_kernel<DT, true, ZPT>), grid, dim3(256), lds, h->stream, a, M, S,
+                         R, CT);
+    else
+      hipLaunchKernelGGL((bp_lds_pf_kernel<DT, false, ZPT>), grid, dim3(256), lds, h->stream, a, M,
+                         S, R, CT);
+  } else if (h->sym) {
     hipLaunchKernelGGL((bp_lds_kernel<DT, true, ZPT>), grid, dim3(256), lds, h->stream, a, M, S, R, CT);
-  else
+  } else {
     hipLaunchKernelGGL((bp_lds_kernel<DT, false, ZPT>), grid, dim3(256), lds, h->stream, a, M, S, R, CT);
+  }
   return true;
 }
 

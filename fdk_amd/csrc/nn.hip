@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 
 #include "common.hpp"
 
@@ -116,6 +117,87 @@ __global__ __launch_bounds__(256) void conv3d_fwd_kernel(Conv a) {
     }
   }
   conv_epilogue<NT>(a, acc, nz * a.h + y, x0, lane);
+}
+
+// As conv3d_fwd_kernel with a 2x larger wave tile and a software pipeline:
+// one wave owns 2 rows x 32 voxels (four 16-voxel M tiles) x all cout, a block
+// 8 consecutive rows; the A and B fragments of the next (tap, channel chunk)
+// step are loaded while the MFMAs of the current one run.  H % 8 == 0,
+// cin % 32 == 0.
+template <int NT>
+struct Frags {
+  bf16x8 a[4];
+  bf16x8 b[NT];
+};
+
+template <int NT>
+__device__ __forceinline__ void conv2_load(const Conv& a, Frags<NT>& f, int it, int n, int z,
+                                           int y0, int x0, int m, int kq) {
+  const int tap = it % 27, c0 = (it / 27) * 32;
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int zz = z + dz;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const int yy = y0 + (mt >> 1) + dy;
+    const int xx = x0 + 16 * (mt & 1) + m + dx;
+    const bool ok = zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    const long long v = ((((long long)n * a.d + zz) * a.h + yy) * a.w + xx);
+    f.a[mt] = ok ? load8(a.x + v * a.cin + c0 + kq * 8) : zero8();
+  }
+  const u16* wb = a.wt + (long long)tap * a.cout * a.cin + c0 + kq * 8;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) f.b[nt] = load8(wb + (long long)(nt * 16 + m) * a.cin);
+}
+
+template <int NT>
+__device__ __forceinline__ void conv2_mma(f32x4 (&acc)[4][NT], const Frags<NT>& f) {
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(f.a[mt], f.b[nt], acc[mt][nt]);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd2_kernel(Conv a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int x0 = blockIdx.x * 32;
+  const int y0 = blockIdx.y * 8 + 2 * wave;
+  const int nz = blockIdx.z;
+  const int z = nz % a.d, n = nz / a.d;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int niters = 27 * (a.cin / 32);
+  Frags<NT> f0, f1;
+  conv2_load<NT>(a, f0, 0, n, z, y0, x0, m, kq);
+  for (int it = 0; it < niters; it += 2) {
+    if (it + 1 < niters) conv2_load<NT>(a, f1, it + 1, n, z, y0, x0, m, kq);
+    conv2_mma<NT>(acc, f0);
+    if (it + 1 < niters) {
+      if (it + 2 < niters) conv2_load<NT>(a, f0, it + 2, n, z, y0, x0, m, kq);
+      conv2_mma<NT>(acc, f1);
+    }
+  }
+  // epilogue: M tile mt is row y0 + (mt >> 1), voxels x0 + 16 (mt & 1) + 4 kq + i
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + 16 * (mt & 1) + 4 * kq + i;
+        const int co = nt * 16 + m;
+        const long long o = (((long long)nz * a.h + y0 + (mt >> 1)) * a.w + x) * a.cout + co;
+        float v = acc[mt][nt][i];
+        if (a.bias) v += a.bias[co];
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.resid) v += a.resid[o];
+        if (a.out_f32) static_cast<float*>(a.y)[o] = v;
+        else static_cast<u16*>(a.y)[o] = f2bf(v);
+      }
 }
 
 // cin == 1 (the network input): the 27 taps of the single channel form K (one
@@ -508,6 +590,16 @@ extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void*
   return FDK_OK;
 }
 
+// Forward convolution kernel: the pipelined 2-row tile (default) or, with
+// FDK_NN_CONV=1, the 1-row kernel.
+static int conv_kernel_choice() {
+  static const int k = [] {
+    const char* e = std::getenv("FDK_NN_CONV");
+    return (e && e[0] == '1') ? 1 : 2;
+  }();
+  return k;
+}
+
 extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid,
                                  void* y, int n, int d, int h, int wd, int cin, int cout, int relu,
                                  int out_f32, void* stream) {
@@ -516,13 +608,17 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   FDK_REQUIRE(wd % 32 == 0 && h % 4 == 0, "conv3d needs W %% 32 == 0 and H %% 4 == 0 (got %d, %d)", wd, h);
   FDK_REQUIRE(cin == 1 || cin % 32 == 0, "cin must be 1 or a multiple of 32 (got %d)", cin);
   FDK_REQUIRE((long long)n * d <= 65535, "n * d exceeds the grid");
+  FDK_REQUIRE(cout % 16 == 0, "cout %d not a multiple of 16", cout);
   Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
          cin, cout, relu, out_f32};
   dim3 grid(wd / 32, h / 4, n * d);
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool tiled = cin % 32 == 0 && h % 8 == 0 && cout <= 128 && conv_kernel_choice() == 2;
+  dim3 grid2(wd / 32, h / 8, n * d);
 #define FDK_CONV_CASE(NT)                                                                   \
   case NT:                                                                                  \
     if (cin == 1) hipLaunchKernelGGL(conv3d_fwd_c1_kernel<NT>, grid, dim3(256), 0, st, a);  \
+    else if (tiled) hipLaunchKernelGGL(conv3d_fwd2_kernel<NT>, grid2, dim3(256), 0, st, a); \
     else hipLaunchKernelGGL(conv3d_fwd_kernel<NT>, grid, dim3(256), 0, st, a);              \
     break;
   switch (cout / 16) {
@@ -535,7 +631,6 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
       return fdk::fail(FDK_ERR_UNSUPPORTED, "cout %d not in {32, 64, 96, 128, 192}", cout);
   }
 #undef FDK_CONV_CASE
-  if (cout % 16) return fdk::fail(FDK_ERR_UNSUPPORTED, "cout %d not a multiple of 16", cout);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
