@@ -363,9 +363,12 @@ def main():
     b_upd = 4 * s_bytes + 8.0 / B          # 4 taps + volume read/write per 32-view pass
     algo_bytes = b_upd * float(n) ** 3 * B  # per launch (one 32-view pass)
     achieved = algo_bytes / (avg_ms * 1e-3)
-    # the library stages detector strips in LDS (bp_lds_kernel) unless
-    # FDK_BP_KERNEL=v2 selects the direct-gather bp_column_kernel
-    kernel = "bp_column_kernel" if os.environ.get("FDK_BP_KERNEL") == "v2" else "bp_lds_kernel"
+    # the library's choice for this workload: FP32 views use the prefetching
+    # LDS kernel, 16-bit views the LDS kernel; FDK_BP_KERNEL=v2 / v3 force the
+    # direct-gather / non-prefetching kernels
+    forced = os.environ.get("FDK_BP_KERNEL")
+    kernel = ("bp_column_kernel" if forced == "v2" else
+              "bp_lds_kernel" if forced == "v3" or dtype != F.FDK_F32 else "bp_lds_pf_kernel")
     traffic = None
     tpath = os.path.join(HERE, "profiles", f"{kernel}_{args.dtype}_{n}cube_traffic.json")
     if os.path.exists(tpath):

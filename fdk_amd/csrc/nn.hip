@@ -412,6 +412,21 @@ __global__ void upsample_kernel(const u16* x, u16* y, int n, int d, int h, int w
   }
 }
 
+// 16-byte form of upsample_kernel (cx, cy, c0 multiples of 8): block
+// (X-run, Y, n*D+Z), one thread per output voxel and 8-channel chunk.
+__global__ void upsample_v8_kernel(const uint4* x, uint4* y, int d, int h, int w, int cq, int cyq,
+                                   int c0q) {
+  const int D = 2 * d, H = 2 * h, Wd = 2 * w;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= Wd * cq) return;
+  const int X = idx / cq, ch = idx - X * cq;
+  const int Y = blockIdx.y;
+  const int nz = blockIdx.z, Z = nz % D, nn = nz / D;
+  const long long src = (((long long)(nn * d + Z / 2) * h + Y / 2) * w + X / 2) * cq + ch;
+  const long long dst = (((long long)nz * H + Y) * Wd + X) * cyq + c0q + ch;
+  y[dst] = x[src];
+}
+
 __global__ void upsample_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, int w, int cx,
                                     int cy, int c0) {
   const long long total = (long long)n * d * h * w * cx;
@@ -442,6 +457,16 @@ __global__ void channel_copy_kernel(const u16* src, int cs, int src_c0, u16* dst
     const int ch = i % cc;
     dst[v * cd + dst_c0 + ch] = src[v * cs + src_c0 + ch];
   }
+}
+
+// 16-byte form of channel_copy_kernel (all channel counts and offsets
+// multiples of 8, nvox * cc / 8 < 2^31).
+__global__ void channel_copy_v8_kernel(const uint4* src, int csq, int s0q, uint4* dst, int cdq,
+                                       int d0q, int ccq, int total) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const int v = i / ccq, ch = i - v * ccq;
+  dst[(long long)v * cdq + d0q + ch] = src[(long long)v * csq + s0q + ch];
 }
 
 // Output head: out = in + sum_c w[c] h[c] + b (FP32), and its adjoints.
@@ -708,7 +733,15 @@ extern "C" int fdk_nn_upsample2(const void* x, void* y, int n, int d, int h, int
   FDK_REQUIRE(x && y && c0 >= 0 && c0 + cx <= cy, "bad channel range");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const long long small = (long long)n * d * h * w * cx;
-  if (!adjoint)
+  const bool v8 = cx % 8 == 0 && cy % 8 == 0 && c0 % 8 == 0 && (long long)n * 2 * d <= 65535 &&
+                  2 * h <= 65535 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(y) % 16 == 0;
+  if (!adjoint && v8) {
+    const int cq = cx / 8;
+    dim3 grid((2 * w * cq + 255) / 256, 2 * h, n * 2 * d);
+    hipLaunchKernelGGL(upsample_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
+                       static_cast<uint4*>(y), d, h, w, cq, cy / 8, c0 / 8);
+  } else if (!adjoint)
     hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(small * 8)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, cx, cy, c0);
   else
@@ -721,6 +754,16 @@ extern "C" int fdk_nn_upsample2(const void* x, void* y, int n, int d, int h, int
 extern "C" int fdk_nn_channel_copy(const void* src, int cs, int src_c0, void* dst, int cd,
                                    int dst_c0, int cc, int64_t nvox, void* stream) {
   FDK_REQUIRE(src && dst && src_c0 + cc <= cs && dst_c0 + cc <= cd, "bad channel range");
+  if (cs % 8 == 0 && src_c0 % 8 == 0 && cd % 8 == 0 && dst_c0 % 8 == 0 && cc % 8 == 0 &&
+      nvox * (cc / 8) < (1LL << 31) && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
+      reinterpret_cast<uintptr_t>(dst) % 16 == 0) {
+    const int total = (int)(nvox * (cc / 8));
+    hipLaunchKernelGGL(channel_copy_v8_kernel, dim3((total + 255) / 256), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const uint4*>(src), cs / 8,
+                       src_c0 / 8, static_cast<uint4*>(dst), cd / 8, dst_c0 / 8, cc / 8, total);
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
   hipLaunchKernelGGL(channel_copy_kernel, dim3(blocks_for(nvox * cc)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<const u16*>(src), cs, src_c0,
                      static_cast<u16*>(dst), cd, dst_c0, cc, (long long)nvox);

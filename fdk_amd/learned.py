@@ -201,6 +201,21 @@ def infer_tiled(net: UNet3D, vol, out=None, batch: int = 4):
     return out
 
 
+def infer_slabs(net: UNet3D, vol, out=None, slab: int = PATCH, overlap: int = OVERLAP):
+    """Apply the network to a (D, H, W) FP32 device volume in z-slabs of whole
+    x-y slices (slab deep, neighbouring slabs overlapping by ``overlap`` and
+    split at the midpoint), batch 1: 1.125x redundant work at 1024^3 against
+    1.43x for 128^3 tiles.  D >= slab; H, W as UNet3D requires."""
+    import torch
+    D = vol.shape[0]
+    out = torch.empty_like(vol) if out is None else out
+    for sz, z0, z1 in tiles(D, slab, overlap):
+        res = net.forward(vol[sz:sz + slab].unsqueeze(0))
+        out[z0:z1].copy_(res[0, z0 - sz:z1 - sz])
+        del res
+    return out
+
+
 def rmse(a, b) -> float:
     import torch
     return float(torch.sqrt(torch.mean((a - b).double() ** 2)).item())
@@ -250,8 +265,10 @@ def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=
     sparse = scanner.fdk(raw, sparse_step, None, store)
     torch.cuda.synchronize()
     t_sparse = time.perf_counter() - t0
+    net.forward(sparse[:PATCH].unsqueeze(0))  # warm the whole-slice shapes once
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    learned = infer_tiled(net, sparse)
+    learned = infer_slabs(net, sparse)
     torch.cuda.synchronize()
     t_net = time.perf_counter() - t0
 
