@@ -513,6 +513,13 @@ __device__ __forceinline__ void strip_commit(const typename Quad<DT>::raw (&pre)
   }
 }
 
+// floor(v) as an int and v - floor(v), one instruction each.
+__device__ __forceinline__ int floor_int(float v) {
+  int iv;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iv) : "v"(v));
+  return iv;
+}
+
 // The taps of one view from the staged strip: a tap index iv reads
 // row[iv - c0] (a sample inside the view always lies in the staged columns),
 // any other sample the zero slot.
@@ -530,10 +537,9 @@ __device__ __forceinline__ void compute_lds(float (&acc)[ZPT], float (&accm)[ZPT
   for (int j = 0; j < ZPT; ++j) {
     {
       const float v = fmaf(zt[j], c.dV, c.Vb);
-      const float fv = floorf(v);
-      const int iv = (int)fv;
+      const int iv = floor_int(v);
       const bool ok = c.uok && (unsigned)iv <= (unsigned)(W - 2);
-      const float t = v - fv;
+      const float t = __builtin_amdgcn_fractf(v);
       const float* p = rowt + (ok ? iv : zt_idx);
       const float ra = fmaf(t, p[1] - p[0], p[0]);
       const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
@@ -541,10 +547,9 @@ __device__ __forceinline__ void compute_lds(float (&acc)[ZPT], float (&accm)[ZPT
     }
     if (SYM) {
       const float v = fmaf(zm[j], c.dV, c.Vb);
-      const float fv = floorf(v);
-      const int iv = (int)fv;
+      const int iv = floor_int(v);
       const bool ok = c.uok && (unsigned)iv <= (unsigned)(W - 2);
-      const float t = v - fv;
+      const float t = __builtin_amdgcn_fractf(v);
       const float* p = rowm + (ok ? iv : zm_idx);
       const float ra = fmaf(t, p[1] - p[0], p[0]);
       const float rb = fmaf(t, p[CT + 1] - p[CT], p[CT]);
@@ -676,8 +681,8 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
   FDK_LDS_PROLOGUE
   if (a.nproj <= 0) return;
   {
-    const ColView c = col_view(M.m[0], x, y, a.H, active);
-    if (lane == 0) footprint_put<SYM>(info[0], wave, c, zt0, zt1, zm0, zm1);
+    const ColView c0v = col_view(M.m[0], x, y, a.H, active);
+    if (lane == 0) footprint_put<SYM>(info[0], wave, c0v, zt0, zt1, zm0, zm1);
   }
   __syncthreads();
   Strip cur = strip_get<SYM>(info[0], a.W, R, CT);
@@ -698,11 +703,11 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
       nxt = strip_get<SYM>(info[(s + 1) & 1], a.W, R, CT);
       if (nxt.fits) strip_issue<DT, SYM>(pre, nxt, view_rsrc(a, s + 1), a.W, CT, stage, stm, wave, lane);
     }
-    const ColView c = col_view(M.m[s], x, y, a.H, active);
+    const ColView cv = col_view(M.m[s], x, y, a.H, active);
     if (cur.fits)
-      compute_lds<SYM, ZPT>(acc, accm, zt, zm, c, cur, stage, stm, a.W, R, CT);
-    else if (c.uok)
-      compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, c, a,
+      compute_lds<SYM, ZPT>(acc, accm, zt, zm, cv, cur, stage, stm, a.W, R, CT);
+    else if (cv.uok)
+      compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, cv, a,
                                    static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
                                    S.so[s][0], S.so[s][1]);
     __syncthreads();  // every wave is done with strip s

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -98,15 +99,216 @@ __device__ void fft_lds(float2* buf, const float2* __restrict__ tw, int N, int l
   }
 }
 
+// ---------------------------------------------------------------- Stockham FFT
+// Self-sorting mixed-radix FFT in LDS (natural order in and out): each pass
+// reads R values per butterfly (stride N/R), twiddles them, runs a radix-R DFT
+// in registers and writes them back at stride Ns, the product of the radices
+// before it.  Radix-16/8 passes cut the LDS round trips per transform to 3-4
+// (against log4 N in-place passes with a bit-reversed entry), and the filter
+// multiply rides on the first inverse pass's reads.
+
+// v * (sign * i): sign -1 multiplies by -i, +1 by +i.
+__device__ __forceinline__ float2 mul_si(float2 v, float sign) {
+  return sign < 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// X[k] = sum_n v[n] e^{sign 2 pi i n k / R}
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  const float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3, float sign) {
+  const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+  const float2 a3 = mul_si(csub(v1, v3), sign);
+  v0 = cadd(a0, a2);
+  v2 = csub(a0, a2);
+  v1 = cadd(a1, a3);
+  v3 = csub(a1, a3);
+}
+
+// e^{sign 2 pi i p / 16}, p = 0..9
+__device__ __forceinline__ float2 w16(int p, float sign) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, c2 = 0.70710678118654752f;
+  float2 w;
+  switch (p) {
+    case 0: w = make_float2(1.f, 0.f); break;
+    case 1: w = make_float2(c1, s1); break;
+    case 2: w = make_float2(c2, c2); break;
+    case 3: w = make_float2(s1, c1); break;
+    case 4: w = make_float2(0.f, 1.f); break;
+    case 5: w = make_float2(-s1, c1); break;
+    case 6: w = make_float2(-c2, c2); break;
+    case 7: w = make_float2(-c1, s1); break;
+    case 8: w = make_float2(-1.f, 0.f); break;
+    default: w = make_float2(-c1, -s1); break;  // 9
+  }
+  w.y *= sign;
+  return w;
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 (&v)[R], float sign);
+
+template <>
+__device__ __forceinline__ void dft<2>(float2 (&v)[2], float) {
+  dft2(v[0], v[1]);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2 (&v)[4], float sign) {
+  dft4(v[0], v[1], v[2], v[3], sign);
+}
+
+template <>
+__device__ __forceinline__ void dft<8>(float2 (&v)[8], float sign) {
+  // even / odd halves, then X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k]
+  dft4(v[0], v[2], v[4], v[6], sign);
+  dft4(v[1], v[3], v[5], v[7], sign);
+  float2 o[4] = {v[1], cmul(v[3], w16(2, sign)), mul_si(v[5], sign), cmul(v[7], w16(6, sign))};
+  const float2 e[4] = {v[0], v[2], v[4], v[6]};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <>
+__device__ __forceinline__ void dft<16>(float2 (&v)[16], float sign) {
+  // F_e = DFT4(v[e], v[e+4], v[e+8], v[e+12]); X[k'+4q] = DFT4_e(W16^{e k'} F_e[k'])[q]
+#pragma unroll
+  for (int e = 0; e < 4; ++e) dft4(v[e], v[e + 4], v[e + 8], v[e + 12], sign);
+  // v[e + 4k'] now holds F_e[k']
+#pragma unroll
+  for (int e = 1; e < 4; ++e)
+#pragma unroll
+    for (int k = 1; k < 4; ++k) v[e + 4 * k] = cmul(v[e + 4 * k], w16(e * k, sign));
+  float2 x[16];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float2 g0 = v[4 * k], g1 = v[1 + 4 * k], g2 = v[2 + 4 * k], g3 = v[3 + 4 * k];
+    dft4(g0, g1, g2, g3, sign);
+    x[k] = g0;
+    x[k + 4] = g1;
+    x[k + 8] = g2;
+    x[k + 12] = g3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = x[k];
+}
+
+// LDS slot of point i: with PAD, one spare slot after every 16 points, so the
+// stride-16 writes of the first pass (and the 2 KiB-apart runs of the second)
+// spread over the banks instead of landing on one.
+template <bool PAD>
+__device__ __forceinline__ int sk_slot(int i) {
+  return PAD ? i + (i >> 4) : i;
+}
+
+// One Stockham pass over N points in buf with BPT butterflies per thread;
+// H (optional) multiplies the values as they are read.
+template <int R, int BPT, bool PAD>
+__device__ __forceinline__ void sk_pass(float2* buf, int N, int Ns, float sign,
+                                        const float2* __restrict__ tw,
+                                        const float* __restrict__ H) {
+  const int NB = N / R;
+  float2 v[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int j = threadIdx.x + q * 256;
+    if (j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = buf[sk_slot<PAD>(j + r * NB)];
+        if (H) {
+          const float h = H[j + r * NB];
+          x = make_float2(x.x * h, x.y * h);
+        }
+        v[q][r] = x;
+      }
+      if (Ns > 1) {
+        const int step = (j & (Ns - 1)) * (N / (Ns * R));
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[step * r];  // e^{-2 pi i step r / N}
+          w.y *= -sign;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft<R>(v[q], sign);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int j = threadIdx.x + q * 256;
+    if (j < NB) {
+      const int k = j & (Ns - 1);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[sk_slot<PAD>(base + r * Ns)] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Radix plans for N = 2^M, 9 <= M <= 13 (256 threads); the LDS is padded for
+// M <= 12 (N + N/16 slots; at M = 13 that would pass 64 KiB).
+template <int M>
+constexpr bool sk_pad() {
+  return M <= 12;
+}
+
+template <int M>
+__device__ __forceinline__ void fft_sk(float2* buf, const float2* tw, float sign, const float* H) {
+  constexpr int N = 1 << M;
+  constexpr bool P = sk_pad<M>();
+  if constexpr (M == 9) {
+    sk_pass<8, 1, P>(buf, N, 1, sign, tw, H);
+    sk_pass<8, 1, P>(buf, N, 8, sign, tw, nullptr);
+    sk_pass<8, 1, P>(buf, N, 64, sign, tw, nullptr);
+  } else if constexpr (M == 10) {
+    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
+    sk_pass<8, 1, P>(buf, N, 16, sign, tw, nullptr);
+    sk_pass<8, 1, P>(buf, N, 128, sign, tw, nullptr);
+  } else if constexpr (M == 11) {
+    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
+    sk_pass<16, 1, P>(buf, N, 16, sign, tw, nullptr);
+    sk_pass<8, 1, P>(buf, N, 256, sign, tw, nullptr);
+  } else if constexpr (M == 12) {
+    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
+    sk_pass<16, 1, P>(buf, N, 16, sign, tw, nullptr);
+    sk_pass<16, 1, P>(buf, N, 256, sign, tw, nullptr);
+  } else {
+    static_assert(M == 13, "plans cover 2^9 .. 2^13");
+    sk_pass<16, 2, P>(buf, N, 1, sign, tw, H);
+    sk_pass<16, 2, P>(buf, N, 16, sign, tw, nullptr);
+    sk_pass<16, 2, P>(buf, N, 256, sign, tw, nullptr);
+    sk_pass<2, 16, P>(buf, N, 4096, sign, tw, nullptr);
+  }
+}
+
 __device__ inline unsigned f2ord(float f) {
   const unsigned u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// M = log2 N selects the Stockham plan (9..13); M = 0 runs the in-place
+// bit-reversed radix-2^2 FFT for the other lengths.
+template <int M>
 __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
   extern __shared__ float2 buf[];
   const int view = blockIdx.y;
-  const int ra = 2 * blockIdx.x, rb = ra + 1;
+  // Row pairs remapped so that each XCD (hardware block b -> XCD b % 8) filters
+  // a contiguous run of rows: the 8-byte transposed stores of neighbouring
+  // pairs then meet in one L2 and leave it as whole lines.
+  const unsigned gx = gridDim.x;
+  const unsigned bx = (gx % 8u == 0u) ? (blockIdx.x % 8u) * (gx / 8u) + blockIdx.x / 8u : blockIdx.x;
+  const int ra = 2 * (int)bx, rb = ra + 1;
   const bool has_b = rb < a.nrows;
   const float* sa = a.src + view * a.src_view + (long long)ra * a.src_pitch;
   const float* sb = sa + a.src_pitch;
@@ -140,39 +342,75 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
       rrb = (float)(((double)xb[4] + xb[5] + xb[6]) / 3);
     }
   }
-  for (int n = threadIdx.x; n < a.N; n += blockDim.x) {
-    const int p = n - off;
-    float va, vb;
-    if (p < 0) { va = la; vb = lb; }
-    else if (p >= nu) { va = rra; vb = rrb; }
-    else {
-      va = sa[p] * (ca ? ca[p] : 1.f);
-      vb = has_b ? sb[p] * (cb ? cb[p] : 1.f) : 0.f;
+  if constexpr (M != 0) {
+    // Every load of the thread's E points issued before any is used (clamped
+    // addresses, pads selected afterwards), so one memory latency covers them.
+    constexpr int E = (1 << M) / 256;
+    const float* sbb = has_b ? sb : sa;
+    const float* cbb = has_b ? cb : ca;
+    float xa[E], xb[E], wa[E], wb[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int p = min(max(threadIdx.x + 256 * i - off, 0), nu - 1);
+      xa[i] = sa[p];
+      xb[i] = sbb[p];
+      wa[i] = ca ? ca[p] : 1.f;
+      wb[i] = cbb ? cbb[p] : 1.f;
     }
-    buf[brev_n(n, a.logN)] = make_float2(va, vb);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int n = threadIdx.x + 256 * i;
+      const int p = n - off;
+      float va = xa[i] * wa[i], vb = has_b ? xb[i] * wb[i] : 0.f;
+      if (p < 0) {
+        va = la;
+        vb = lb;
+      } else if (p >= nu) {
+        va = rra;
+        vb = rrb;
+      }
+      buf[sk_slot<sk_pad<M>()>(n)] = make_float2(va, vb);
+    }
+  } else {
+    for (int n = threadIdx.x; n < a.N; n += blockDim.x) {
+      const int p = n - off;
+      float va, vb;
+      if (p < 0) { va = la; vb = lb; }
+      else if (p >= nu) { va = rra; vb = rrb; }
+      else {
+        va = sa[p] * (ca ? ca[p] : 1.f);
+        vb = has_b ? sb[p] * (cb ? cb[p] : 1.f) : 0.f;
+      }
+      buf[brev_n(n, a.logN)] = make_float2(va, vb);
+    }
   }
   __syncthreads();
-  fft_lds(buf, a.tw, a.N, a.logN, -1.f);
-  // Filter multiply fused with the bit-reversal permutation of the inverse pass.
-  for (int k = threadIdx.x; k < a.N; k += blockDim.x) {
-    const unsigned r = brev_n(k, a.logN);
-    if ((unsigned)k < r) {
-      const float2 xk = buf[k], xr = buf[r];
-      const float hk = a.H[k], hr = a.H[r];
-      buf[k] = make_float2(xr.x * hr, xr.y * hr);
-      buf[r] = make_float2(xk.x * hk, xk.y * hk);
-    } else if ((unsigned)k == r) {
-      const float hk = a.H[k];
-      buf[k] = make_float2(buf[k].x * hk, buf[k].y * hk);
+  if constexpr (M != 0) {
+    fft_sk<M>(buf, a.tw, -1.f, nullptr);
+    fft_sk<M>(buf, a.tw, 1.f, a.H);  // the filter multiply rides on the first inverse pass
+  } else {
+    fft_lds(buf, a.tw, a.N, a.logN, -1.f);
+    // Filter multiply fused with the bit-reversal permutation of the inverse pass.
+    for (int k = threadIdx.x; k < a.N; k += blockDim.x) {
+      const unsigned r = brev_n(k, a.logN);
+      if ((unsigned)k < r) {
+        const float2 xk = buf[k], xr = buf[r];
+        const float hk = a.H[k], hr = a.H[r];
+        buf[k] = make_float2(xr.x * hr, xr.y * hr);
+        buf[r] = make_float2(xk.x * hk, xk.y * hk);
+      } else if ((unsigned)k == r) {
+        const float hk = a.H[k];
+        buf[k] = make_float2(buf[k].x * hk, buf[k].y * hk);
+      }
     }
+    __syncthreads();
+    fft_lds(buf, a.tw, a.N, a.logN, 1.f);
   }
-  __syncthreads();
-  fft_lds(buf, a.tw, a.N, a.logN, 1.f);
   const float inv = 1.0f / (float)a.N;
   float* d = a.dst + view * a.dst_view;
   float lo = INFINITY, hi = -INFINITY;
   for (int u = threadIdx.x; u < nu; u += blockDim.x) {
-    const float2 c = buf[off + u];
+    const float2 c = buf[M ? sk_slot<sk_pad<M>()>(off + u) : off + u];
     const float fa = c.x * inv, fb = c.y * inv;
     if (a.transpose) {
       float* o = d + (long long)u * a.nrows + ra;
@@ -260,8 +498,8 @@ int twiddles(int N, const float2** out) {
     *out = it->second;
     return FDK_OK;
   }
-  std::vector<float2> h(N / 2);
-  for (int k = 0; k < N / 2; ++k) {
+  std::vector<float2> h(N);  // the Stockham passes index the whole circle
+  for (int k = 0; k < N; ++k) {
     const double a = -2.0 * 3.141592653589793238462 * k / N;
     h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
@@ -286,6 +524,16 @@ int log2_exact(int N) {
   return (1 << m) == N ? m : -1;
 }
 
+// 1: Stockham FFT where a plan exists (default); FDK_FILTER_FFT=inplace keeps
+// the in-place radix-2^2 FFT for every length.
+int fft_choice() {
+  static const int k = [] {
+    const char* e = std::getenv("FDK_FILTER_FFT");
+    return (e && std::strcmp(e, "inplace") == 0) ? 0 : 1;
+  }();
+  return k;
+}
+
 int launch_filter(FilterArgs& a, int nviews, hipStream_t st) {
   FDK_REQUIRE(a.N >= 16 && a.N <= 16384 && log2_exact(a.N) > 0, "filter length %d must be a power of two in 16..16384", a.N);
   FDK_REQUIRE(a.nu >= 4 && a.nu <= a.N, "row length %d must be in 4..N (%d)", a.nu, a.N);
@@ -295,9 +543,17 @@ int launch_filter(FilterArgs& a, int nviews, hipStream_t st) {
   if (nviews == 0) return FDK_OK;
   a.logN = log2_exact(a.N);
   if (int r = twiddles(a.N, &a.tw)) return r;
-  const size_t lds = sizeof(float2) * a.N;
+  const int plan = (fft_choice() && a.logN >= 9 && a.logN <= 13) ? a.logN : 0;
+  const size_t lds = sizeof(float2) * (plan >= 9 && plan <= 12 ? a.N + a.N / 16 : a.N);
   dim3 grid((a.nrows + 1) / 2, nviews);
-  hipLaunchKernelGGL(filter_rows_kernel, grid, dim3(256), lds, st, a);
+  switch (plan) {
+    case 9: hipLaunchKernelGGL(filter_rows_kernel<9>, grid, dim3(256), lds, st, a); break;
+    case 10: hipLaunchKernelGGL(filter_rows_kernel<10>, grid, dim3(256), lds, st, a); break;
+    case 11: hipLaunchKernelGGL(filter_rows_kernel<11>, grid, dim3(256), lds, st, a); break;
+    case 12: hipLaunchKernelGGL(filter_rows_kernel<12>, grid, dim3(256), lds, st, a); break;
+    case 13: hipLaunchKernelGGL(filter_rows_kernel<13>, grid, dim3(256), lds, st, a); break;
+    default: hipLaunchKernelGGL(filter_rows_kernel<0>, grid, dim3(256), lds, st, a); break;
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

@@ -172,7 +172,8 @@ def test_reconstruct_conebeam_device(torch_cuda):
 
 
 FILTER_CASES = [(256, 64, O.PAD_MEAN4), (100, 37, O.PAD_MEAN3), (2048, 8, O.PAD_MEAN4),
-                (4000, 4, O.PAD_ZERO), (1024, 6, O.PAD_MEAN3), (24, 20, O.PAD_MEAN4)]
+                (4000, 4, O.PAD_ZERO), (1024, 6, O.PAD_MEAN3), (24, 20, O.PAD_MEAN4),
+                (600, 32, O.PAD_MEAN4), (2048, 48, O.PAD_MEAN3)]
 
 
 @pytest.mark.parametrize("nu,nv,pad", FILTER_CASES)
