@@ -200,6 +200,85 @@ __global__ __launch_bounds__(256) void conv3d_fwd2_kernel(Conv a) {
       }
 }
 
+// LDS-staged form for cin % 32 == 0: a block owns 2 planes x 4 rows x 32
+// voxels (one wave: 1 plane x 2 rows, four 16-voxel M tiles, all cout) and,
+// per 32-channel chunk, stages the input halo (4 planes x 6 rows x 34 voxels)
+// in LDS as 16-byte (8-channel) chunks laid out [plane][row][chunk][x], so
+// the 16 lanes of an A fragment read 256 contiguous bytes.  The 27 taps then
+// take A from LDS and B (weights, L2-resident) from memory.  Needs W % 32,
+// H % 4 and D % 2 == 0.
+constexpr int kHaloX = 34, kHaloRows = 6, kHaloPlanes = 4;
+constexpr int kHaloChunks = kHaloPlanes * kHaloRows * 4 * kHaloX;  // 16-byte chunks
+
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
+  __shared__ uint4 halo[kHaloChunks];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 4;
+  const int zb = blockIdx.z * 2;  // n * d + z of the block's first plane
+  const int n = zb / a.d, z0 = zb % a.d;
+  const int pz = wave >> 1, py = 2 * (wave & 1);  // the wave's plane and first row
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < a.cin; c0 += 32) {
+    __syncthreads();  // the previous chunk's reads are done
+    for (int i = threadIdx.x; i < kHaloChunks; i += 256) {
+      const int q = i & 3;
+      int r = i >> 2;
+      const int hx = r % kHaloX;
+      r /= kHaloX;
+      const int hy = r % kHaloRows, hz = r / kHaloRows;
+      const int zz = z0 - 1 + hz, yy = y0 - 1 + hy, xx = x0 - 1 + hx;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+        v = *reinterpret_cast<const uint4*>(
+            a.x + ((((long long)n * a.d + zz) * a.h + yy) * a.w + xx) * a.cin + c0 + 8 * q);
+      halo[((hz * kHaloRows + hy) * 4 + q) * kHaloX + hx] = v;
+    }
+    __syncthreads();
+#pragma unroll 3
+    for (int tap = 0; tap < 27; ++tap) {
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      bf16x8 af[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const int hz = pz + 1 + dz, hy = py + (mt >> 1) + 1 + dy;
+        const int hx = 16 * (mt & 1) + m + 1 + dx;
+        af[mt] = __builtin_bit_cast(bf16x8, halo[((hz * kHaloRows + hy) * 4 + kq) * kHaloX + hx]);
+      }
+      const u16* wb = a.wt + (long long)tap * a.cout * a.cin + c0 + kq * 8;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const bf16x8 bf = load8(wb + (long long)(nt * 16 + m) * a.cin);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(af[mt], bf, acc[mt][nt]);
+      }
+    }
+  }
+  const int zo = blockIdx.z * 2 + pz;  // n * d + z of the wave's plane
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + 16 * (mt & 1) + 4 * kq + i;
+        const int co = nt * 16 + m;
+        const long long o =
+            (((long long)zo * a.h + y0 + py + (mt >> 1)) * a.w + x) * a.cout + co;
+        float v = acc[mt][nt][i];
+        if (a.bias) v += a.bias[co];
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.resid) v += a.resid[o];
+        if (a.out_f32) static_cast<float*>(a.y)[o] = v;
+        else static_cast<u16*>(a.y)[o] = f2bf(v);
+      }
+}
+
 // cin == 1 (the network input): the 27 taps of the single channel form K (one
 // MFMA K step, padded to 32); weights [cout][32].
 template <int NT>
@@ -615,12 +694,15 @@ extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void*
   return FDK_OK;
 }
 
-// Forward convolution kernel: the pipelined 2-row tile (default) or, with
-// FDK_NN_CONV=1, the 1-row kernel.
+// Forward convolution kernel: the LDS-staged halo kernel where its shape
+// rules hold (default), else the pipelined 2-row tile, else the 1-row kernel;
+// FDK_NN_CONV=2 or 1 caps the choice.
 static int conv_kernel_choice() {
   static const int k = [] {
     const char* e = std::getenv("FDK_NN_CONV");
-    return (e && e[0] == '1') ? 1 : 2;
+    if (e && e[0] == '1') return 1;
+    if (e && e[0] == '2') return 2;
+    return 3;
   }();
   return k;
 }
@@ -638,11 +720,15 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
          cin, cout, relu, out_f32};
   dim3 grid(wd / 32, h / 4, n * d);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool tiled = cin % 32 == 0 && h % 8 == 0 && cout <= 128 && conv_kernel_choice() == 2;
+  const int ck = conv_kernel_choice();
+  const bool staged = cin % 32 == 0 && d % 2 == 0 && cout <= 128 && ck == 3;
+  const bool tiled = !staged && cin % 32 == 0 && h % 8 == 0 && cout <= 128 && ck >= 2;
   dim3 grid2(wd / 32, h / 8, n * d);
+  dim3 grid3(wd / 32, h / 4, n * d / 2);
 #define FDK_CONV_CASE(NT)                                                                   \
   case NT:                                                                                  \
     if (cin == 1) hipLaunchKernelGGL(conv3d_fwd_c1_kernel<NT>, grid, dim3(256), 0, st, a);  \
+    else if (staged) hipLaunchKernelGGL(conv3d_fwd3_kernel<NT>, grid3, dim3(256), 0, st, a); \
     else if (tiled) hipLaunchKernelGGL(conv3d_fwd2_kernel<NT>, grid2, dim3(256), 0, st, a); \
     else hipLaunchKernelGGL(conv3d_fwd_kernel<NT>, grid, dim3(256), 0, st, a);              \
     break;
