@@ -211,6 +211,31 @@ constexpr int kHaloX = 34, kHaloRows = 6, kHaloPlanes = 4;
 constexpr int kHaloChunks = kHaloPlanes * kHaloRows * 4 * kHaloX;  // 16-byte chunks
 
 template <int NT>
+__device__ __forceinline__ void fwd3_load_b(bf16x8 (&b)[NT], const u16* wc, int tap,
+                                            long long tap_stride, int cin) {
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) b[nt] = load8(wc + tap * tap_stride + (long long)nt * 16 * cin);
+}
+
+template <int NT>
+__device__ __forceinline__ void fwd3_tap(f32x4 (&acc)[4][NT], const uint4* halo,
+                                         const bf16x8 (&b)[NT], int tap, int pz, int py, int m,
+                                         int kq) {
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  bf16x8 af[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const int hz = pz + 1 + dz, hy = py + (mt >> 1) + 1 + dy;
+    const int hx = 16 * (mt & 1) + m + 1 + dx;
+    af[mt] = __builtin_bit_cast(bf16x8, halo[((hz * kHaloRows + hy) * 4 + kq) * kHaloX + hx]);
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(af[mt], b[nt], acc[mt][nt]);
+}
+
+template <int NT>
 __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
   __shared__ uint4 halo[kHaloChunks];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -240,23 +265,19 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
       halo[((hz * kHaloRows + hy) * 4 + q) * kHaloX + hx] = v;
     }
     __syncthreads();
-#pragma unroll 3
-    for (int tap = 0; tap < 27; ++tap) {
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-      bf16x8 af[4];
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const int hz = pz + 1 + dz, hy = py + (mt >> 1) + 1 + dy;
-        const int hx = 16 * (mt & 1) + m + 1 + dx;
-        af[mt] = __builtin_bit_cast(bf16x8, halo[((hz * kHaloRows + hy) * 4 + kq) * kHaloX + hx]);
-      }
-      const u16* wb = a.wt + (long long)tap * a.cout * a.cin + c0 + kq * 8;
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const bf16x8 bf = load8(wb + (long long)(nt * 16 + m) * a.cin);
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(af[mt], bf, acc[mt][nt]);
-      }
+    // weight fragments run two taps ahead of the MFMAs in a 3-deep register ring
+    const u16* wc = a.wt + c0 + kq * 8 + (long long)m * a.cin;
+    const long long tap_stride = (long long)a.cout * a.cin;
+    bf16x8 b0[NT], b1[NT], b2[NT];
+    fwd3_load_b<NT>(b0, wc, 0, tap_stride, a.cin);
+    fwd3_load_b<NT>(b1, wc, 1, tap_stride, a.cin);
+    for (int t3 = 0; t3 < 27; t3 += 3) {
+      fwd3_load_b<NT>(b2, wc, t3 + 2, tap_stride, a.cin);
+      fwd3_tap<NT>(acc, halo, b0, t3, pz, py, m, kq);
+      if (t3 + 3 < 27) fwd3_load_b<NT>(b0, wc, t3 + 3, tap_stride, a.cin);
+      fwd3_tap<NT>(acc, halo, b1, t3 + 1, pz, py, m, kq);
+      if (t3 + 4 < 27) fwd3_load_b<NT>(b1, wc, t3 + 4, tap_stride, a.cin);
+      fwd3_tap<NT>(acc, halo, b2, t3 + 2, pz, py, m, kq);
     }
   }
   const int zo = blockIdx.z * 2 + pz;  // n * d + z of the wave's plane
