@@ -480,6 +480,34 @@ __global__ void avgpool_kernel(const u16* x, u16* y, int n, int d, int h, int w,
   }
 }
 
+// 16-byte form of avgpool_kernel (c % 8 == 0): block (x-run, yo, n*Do+zo),
+// one thread per output voxel and 8-channel chunk.
+__global__ void avgpool_v8_kernel(const uint4* x, uint4* y, int d, int h, int w, int cq) {
+  const int Do = d / 2, Ho = h / 2, Wo = w / 2;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= Wo * cq) return;
+  const int xo = idx / cq, ch = idx - xo * cq;
+  const int yo = blockIdx.y;
+  const int nz = blockIdx.z, zo = nz % Do, nn = nz / Do;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int zz = 2 * zo + (q >> 2), yy = 2 * yo + ((q >> 1) & 1), xx = 2 * xo + (q & 1);
+    const uint4 v = x[((((long long)nn * d + zz) * h + yy) * w + xx) * cq + ch];
+    const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc[2 * e] += __uint_as_float(u[e] << 16);
+      acc[2 * e + 1] += __uint_as_float(u[e] & 0xffff0000u);
+    }
+  }
+  unsigned o[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    o[e] = (unsigned)f2bf(0.125f * acc[2 * e]) | ((unsigned)f2bf(0.125f * acc[2 * e + 1]) << 16);
+  y[(((long long)nz * Ho + yo) * Wo + xo) * cq + ch] = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
 __global__ void avgpool_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, int w, int c) {
   const long long total = (long long)n * d * h * w * c;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
@@ -577,6 +605,28 @@ __global__ void head_fwd_kernel(const u16* h, const float* w, const float* b, co
     for (int ch = 0; ch < c; ++ch) s = fmaf(w[ch], bf2f(h[v * c + ch]), s);
     out[v] = in[v] + s;
   }
+}
+
+// 16-byte form of head_fwd_kernel for C channels (C % 8 == 0): one voxel per
+// thread, its C BF16 activations in C / 8 loads; w and b are wave-uniform.
+template <int C>
+__global__ __launch_bounds__(256) void head_fwd_v8_kernel(const uint4* h, const float* w,
+                                                          const float* b, const float* in,
+                                                          float* out, long long nvox) {
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvox) return;
+  float s = b[0];
+#pragma unroll
+  for (int q = 0; q < C / 8; ++q) {
+    const uint4 x = h[v * (C / 8) + q];
+    const unsigned u[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      s = fmaf(w[8 * q + 2 * e], __uint_as_float(u[e] << 16), s);
+      s = fmaf(w[8 * q + 2 * e + 1], __uint_as_float(u[e] & 0xffff0000u), s);
+    }
+  }
+  out[v] = in[v] + s;
 }
 
 __global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w, u16* dh,
@@ -825,7 +875,13 @@ extern "C" int fdk_nn_avgpool2(const void* x, void* y, int n, int d, int h, int 
   FDK_REQUIRE(x && y && d % 2 == 0 && h % 2 == 0 && w % 2 == 0, "avgpool needs even sizes");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const long long full = (long long)n * d * h * w * c;
-  if (!adjoint)
+  if (!adjoint && c % 8 == 0 && (long long)n * (d / 2) <= 65535 && h / 2 <= 65535 &&
+      reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0) {
+    const int cq = c / 8;
+    dim3 grid(((w / 2) * cq + 255) / 256, h / 2, n * (d / 2));
+    hipLaunchKernelGGL(avgpool_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
+                       static_cast<uint4*>(y), d, h, w, cq);
+  } else if (!adjoint)
     hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for(full / 8)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, c);
   else
@@ -881,6 +937,13 @@ extern "C" int fdk_nn_channel_copy(const void* src, int cs, int src_c0, void* ds
 extern "C" int fdk_nn_head(const void* h, const float* w, const float* b, const float* in,
                            float* out, int64_t nvox, int c, void* stream) {
   FDK_REQUIRE(h && w && b && in && out && c <= 63, "bad head arguments");
+  if (c == 32 && reinterpret_cast<uintptr_t>(h) % 16 == 0) {
+    hipLaunchKernelGGL(head_fwd_v8_kernel<32>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const uint4*>(h), w, b, in,
+                       out, (long long)nvox);
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
   hipLaunchKernelGGL(head_fwd_kernel, dim3(blocks_for(nvox)), dim3(256), 0,
                      static_cast<hipStream_t>(stream), static_cast<const u16*>(h), w, b, in, out,
                      (long long)nvox, c);
