@@ -217,18 +217,20 @@ __device__ __forceinline__ void fwd3_load_b(bf16x8 (&b)[NT], const u16* wc, int 
   for (int nt = 0; nt < NT; ++nt) b[nt] = load8(wc + tap * tap_stride + (long long)nt * 16 * cin);
 }
 
-template <int NT>
-__device__ __forceinline__ void fwd3_tap(f32x4 (&acc)[4][NT], const uint4* halo,
-                                         const bf16x8 (&b)[NT], int tap, int pz, int py, int m,
-                                         int kq) {
+__device__ __forceinline__ void fwd3_load_a(bf16x8 (&af)[4], const uint4* halo, int tap, int pz,
+                                            int py, int m, int kq) {
   const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-  bf16x8 af[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const int hz = pz + 1 + dz, hy = py + (mt >> 1) + 1 + dy;
     const int hx = 16 * (mt & 1) + m + 1 + dx;
     af[mt] = __builtin_bit_cast(bf16x8, halo[((hz * kHaloRows + hy) * 4 + kq) * kHaloX + hx]);
   }
+}
+
+template <int NT>
+__device__ __forceinline__ void fwd3_mma(f32x4 (&acc)[4][NT], const bf16x8 (&af)[4],
+                                         const bf16x8 (&b)[NT]) {
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
@@ -271,13 +273,22 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
     bf16x8 b0[NT], b1[NT], b2[NT];
     fwd3_load_b<NT>(b0, wc, 0, tap_stride, a.cin);
     fwd3_load_b<NT>(b1, wc, 1, tap_stride, a.cin);
+    // Scheduling barriers keep each weight load where it is issued, two taps
+    // ahead of its MFMAs, so its latency runs under the taps between.
+    bf16x8 af[4];
     for (int t3 = 0; t3 < 27; t3 += 3) {
       fwd3_load_b<NT>(b2, wc, t3 + 2, tap_stride, a.cin);
-      fwd3_tap<NT>(acc, halo, b0, t3, pz, py, m, kq);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd3_load_a(af, halo, t3, pz, py, m, kq);
+      fwd3_mma<NT>(acc, af, b0);
       if (t3 + 3 < 27) fwd3_load_b<NT>(b0, wc, t3 + 3, tap_stride, a.cin);
-      fwd3_tap<NT>(acc, halo, b1, t3 + 1, pz, py, m, kq);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd3_load_a(af, halo, t3 + 1, pz, py, m, kq);
+      fwd3_mma<NT>(acc, af, b1);
       if (t3 + 4 < 27) fwd3_load_b<NT>(b1, wc, t3 + 4, tap_stride, a.cin);
-      fwd3_tap<NT>(acc, halo, b2, t3 + 2, pz, py, m, kq);
+      __builtin_amdgcn_sched_barrier(0);
+      fwd3_load_a(af, halo, t3 + 2, pz, py, m, kq);
+      fwd3_mma<NT>(acc, af, b2);
     }
   }
   const int zo = blockIdx.z * 2 + pz;  // n * d + z of the wave's plane
