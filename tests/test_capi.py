@@ -143,3 +143,18 @@ def test_random_ellipsoids_are_seeded_and_inside():
         R = np.asarray(e["rot"])
         assert np.allclose(R @ R.T, np.eye(3), atol=1e-12)
         assert -0.5 <= e["density"] <= 1.0
+
+
+def test_pipeline_rejects_bad_arguments_without_a_device():
+    L = F.lib()
+    g = F.Parameter(nx=16, ny=16, nz=16, nu=24, nv=20, nproj=8).c()
+    h = C.c_void_p()
+    top = (C.c_int64 * 6)(0, 0, 0, 16, 16, 16)
+    rc, msg = _status(L.fdk_pipe_create(C.byref(g), top, None, 7, F.PAD_MEAN4, 2, C.byref(h)))
+    assert rc == -1 and "store dtype" in msg and not h.value
+    rc, msg = _status(L.fdk_pipe_create(C.byref(g), top, None, F.FDK_F32, F.PAD_MEAN4, 9, C.byref(h)))
+    assert rc == -1 and "slots" in msg and not h.value
+    rc, msg = _status(L.fdk_pipe_push_raw(None, None, 4, 0, 1))
+    assert rc == -1 and "null" in msg
+    rc, msg = _status(L.fdk_pipe_finish(None))
+    assert rc == -1
