@@ -142,6 +142,44 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
                                     "sources" if use_ref else "oracle port (FP64)"))}
 
 
+def fp_stage_c1():
+    """Ray-marching forward projection at the 256^3 / 360-view geometry (1 mm
+    voxels, 256^2 detector of 1.6 mm, SOD 1000, SDD 1500) with a 0.5 mm step,
+    clamp-to-edge: samples/s, timed with events over 3 repetitions.  The sample
+    count is the projection of a ones volume divided by the step (each sample
+    adds step x value)."""
+    import numpy as np
+    import torch
+    import fdk_amd as F
+    p = F.Parameter(nx=256, ny=256, nz=256, dx=1.0, dy=1.0, dz=1.0, nu=256, nv=256, du=1.6, dv=1.6,
+                    sod=1000.0, sid=1500.0, nproj=360, volume_position_angle_rad=0.0,
+                    filter_type=5)
+    step = 0.5
+    vol = torch.ones((256, 256, 256), device="cuda")
+    out = torch.empty((360, 256, 256), device="cuda")
+    F.forward_project(p, vol, 0, 360, out, step, F.FP_CLAMP)
+    torch.cuda.synchronize()
+    samples = float(out.double().sum().item()) / step
+    ell = F.random_ellipsoids(10, seed=1, radius_mm=120.0, half_height_mm=120.0)
+    F.phantom_rasterize(ell, p, vol)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    F.forward_project(p, vol, 0, 360, out, step, F.FP_CLAMP)
+    e0.record(st)
+    for _ in range(3):
+        F.forward_project(p, vol, 0, 360, out, step, F.FP_CLAMP)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    rate = samples / (ms * 1e-3)
+    return {"workload": "256^3 volume, 360 views of 256^2, 0.5 mm step, clamp-to-edge trilinear",
+            "ms_per_360_views": round(ms, 3), "samples": int(samples),
+            "gsamples_per_s": round(rate / 1e9, 2),
+            "algorithmic_GBps": round(rate * 32 / 1e9, 1),
+            "frac_of_hbm_peak": round(rate * 32 / HBM_PEAK, 4),
+            "bytes_per_sample": 32}
+
+
 def learned_c4():
     """The learned sparse-view comparison on a held-out phantom: FDK from all
     1024 noisy views of a 1024^3 scan against FDK from every 4th view followed
@@ -337,6 +375,8 @@ def main():
                     "bytes_per_view": p.nu * p.nv * (4 + s_out)}
     log(f"filter stage: {filter_stage}")
     del fout
+    fp_stage = fp_stage_c1()
+    log(f"forward-projection stage: {fp_stage}")
     mats = p.proj_matrices().astype(np.float32)
 
     # ---- the backprojector: mirror slab pair covering the whole volume
@@ -452,6 +492,7 @@ def main():
                      "kernel_ms_avg": round(avg_ms, 3)},
         "cpu_baseline": cpu,
         "filter_stage": filter_stage,
+        "fp_stage": fp_stage,
         "learned": learned,
     }
     print(json.dumps(out), flush=True)
