@@ -90,7 +90,48 @@ __device__ inline float tap_sample(const float* __restrict__ vol, int nx, int ny
   return v0 * (1.f - fz) + v1 * fz;
 }
 
+// tap_sample through a buffer resource over the volume (32-bit offsets, for
+// volumes under 2 GiB): the taps x0 and x1 of each (y, z) row come from one
+// 8-byte load at xb = clamp(ax, 0, nx - 2), each tap picking the element it
+// clamps to, so a sample costs 4 loads instead of 8.  Same values and
+// arithmetic as tap_sample.
+constexpr int kFpRsrcWord3 = 0x00020000;
+
 template <int MODE>
+__device__ inline float tap_sample_buf(__amdgpu_buffer_rsrc_t rv, int nx, int ny, int nz, float px,
+                                       float py, float pz) {
+  const int ax = (int)px, ay = (int)py, az = (int)pz;  // truncation, as the reference
+  const float fx = px - ax, fy = py - ay, fz = pz - az;
+  int y0 = ay, y1 = ay + 1, z0 = az, z1 = az + 1;
+  float mx0 = 1.f, mx1 = 1.f, my0 = 1.f, my1 = 1.f, mz0 = 1.f, mz1 = 1.f;
+  if (MODE != FDK_FP_CLAMP) {
+    mx0 = (ax >= 0 && ax < nx) ? 1.f : 0.f; mx1 = (ax + 1 >= 0 && ax + 1 < nx) ? 1.f : 0.f;
+    my0 = (y0 >= 0 && y0 < ny) ? 1.f : 0.f; my1 = (y1 >= 0 && y1 < ny) ? 1.f : 0.f;
+    mz0 = (z0 >= 0 && z0 < nz) ? 1.f : 0.f; mz1 = (z1 >= 0 && z1 < nz) ? 1.f : 0.f;
+  }
+  y0 = min(max(y0, 0), ny - 1); y1 = min(max(y1, 0), ny - 1);
+  z0 = min(max(z0, 0), nz - 1); z1 = min(max(z1, 0), nz - 1);
+  const int x0 = min(max(ax, 0), nx - 1), x1 = min(max(ax + 1, 0), nx - 1);
+  const int xb = min(max(ax, 0), nx - 2);
+  const bool s0 = x0 != xb, s1 = x1 != xb;  // the tap is the pair's second element
+  const int pz0 = z0 * ny * nx, pz1 = z1 * ny * nx;
+  const int o00 = (pz0 + y0 * nx + xb) * 4, o01 = (pz0 + y1 * nx + xb) * 4;
+  const int o10 = (pz1 + y0 * nx + xb) * 4, o11 = (pz1 + y1 * nx + xb) * 4;
+  const float2 q00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o00, 0, 0));
+  const float2 q01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o01, 0, 0));
+  const float2 q10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o10, 0, 0));
+  const float2 q11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o11, 0, 0));
+  const float t000 = (s0 ? q00.y : q00.x) * (mx0 * my0 * mz0), t001 = (s1 ? q00.y : q00.x) * (mx1 * my0 * mz0);
+  const float t010 = (s0 ? q01.y : q01.x) * (mx0 * my1 * mz0), t011 = (s1 ? q01.y : q01.x) * (mx1 * my1 * mz0);
+  const float t100 = (s0 ? q10.y : q10.x) * (mx0 * my0 * mz1), t101 = (s1 ? q10.y : q10.x) * (mx1 * my0 * mz1);
+  const float t110 = (s0 ? q11.y : q11.x) * (mx0 * my1 * mz1), t111 = (s1 ? q11.y : q11.x) * (mx1 * my1 * mz1);
+  const float gx = 1.f - fx, gy = 1.f - fy;
+  const float v0 = (t000 * gx + t001 * fx) * gy + (t010 * gx + t011 * fx) * fy;
+  const float v1 = (t100 * gx + t101 * fx) * gy + (t110 * gx + t111 * fx) * fy;
+  return v0 * (1.f - fz) + v1 * fz;
+}
+
+template <int MODE, bool BUF>
 __global__ __launch_bounds__(256) void fp_kernel(RayGeom g, const float* __restrict__ vol,
                                                  float step, float* __restrict__ out) {
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -114,8 +155,17 @@ __global__ __launch_bounds__(256) void fp_kernel(RayGeom g, const float* __restr
                 oz = (float)(g.cz - o[2] / g.vz);
     const float sx = (float)(d[0] / g.vx), sy = (float)(-d[1] / g.vy), sz = (float)(-d[2] / g.vz);
     const float tfar = (float)tf;
-    for (float t = (float)tn; MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar; t += step)
-      sum += tap_sample<MODE>(vol, g.nx, g.ny, g.nz, fmaf(sx, t, ox), fmaf(sy, t, oy), fmaf(sz, t, oz));
+    if constexpr (BUF) {
+      const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)vol, 0, g.nx * g.ny * g.nz * 4, kFpRsrcWord3);
+      for (float t = (float)tn; MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar; t += step)
+        sum += tap_sample_buf<MODE>(rv, g.nx, g.ny, g.nz, fmaf(sx, t, ox), fmaf(sy, t, oy),
+                                    fmaf(sz, t, oz));
+    } else {
+      for (float t = (float)tn; MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar; t += step)
+        sum += tap_sample<MODE>(vol, g.nx, g.ny, g.nz, fmaf(sx, t, ox), fmaf(sy, t, oy),
+                                fmaf(sz, t, oz));
+    }
   }
   out[((long long)sv * g.nv + j) * g.nu + i] = sum * step;
 }
@@ -235,10 +285,16 @@ extern "C" int fdk_fp_project(const fdk_geometry* g, const float* dev_vol, int f
   if (count == 0) return FDK_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   dim3 grid((g->nu + 63) / 64, (g->nv + 3) / 4, count);
-  if (mode == FDK_FP_CLAMP)
-    hipLaunchKernelGGL(fp_kernel<FDK_FP_CLAMP>, grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
+  // paired-tap buffer path when the volume's byte offsets fit 32 bits
+  const bool buf = g->nx >= 2 && (long long)g->nx * g->ny * g->nz * 4 < (1LL << 31);
+  if (mode == FDK_FP_CLAMP && buf)
+    hipLaunchKernelGGL((fp_kernel<FDK_FP_CLAMP, true>), grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
+  else if (mode == FDK_FP_CLAMP)
+    hipLaunchKernelGGL((fp_kernel<FDK_FP_CLAMP, false>), grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
+  else if (buf)
+    hipLaunchKernelGGL((fp_kernel<FDK_FP_ZERO, true>), grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
   else
-    hipLaunchKernelGGL(fp_kernel<FDK_FP_ZERO>, grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
+    hipLaunchKernelGGL((fp_kernel<FDK_FP_ZERO, false>), grid, dim3(64, 4), 0, st, r, dev_vol, (float)step_mm, dev_out);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
