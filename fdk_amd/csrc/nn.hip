@@ -387,16 +387,32 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(Wgrad a) {
     const int zz = z + dz, yy = y + dy_;
     const bool zy = zz >= 0 && zz < a.d && yy >= 0 && yy < a.h;
     __syncthreads();
+    // 16-byte loads of 8 channels of one voxel, spread into the transposed
+    // [c][x] rows (consecutive lanes take consecutive x)
     const u16* dyrow = a.dy + (long long)r * W * a.cout;
-    for (int i = threadIdx.x; i < W * a.cout; i += 256) {
-      const int x = i / a.cout, c = i % a.cout;
-      dyT[c * WP + x] = dyrow[i];
+    for (int i = threadIdx.x; i < W * (a.cout / 8); i += 256) {
+      const int q = i / W, x = i - q * W;
+      const uint4 v = *reinterpret_cast<const uint4*>(dyrow + (long long)x * a.cout + 8 * q);
+      const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        dyT[(8 * q + 2 * e) * WP + x] = (u16)(u[e] & 0xffffu);
+        dyT[(8 * q + 2 * e + 1) * WP + x] = (u16)(u[e] >> 16);
+      }
     }
     const u16* xrow = a.x + ((((long long)n * a.d + zz) * a.h + yy) * W) * a.cin;
-    for (int i = threadIdx.x; i < W * a.cin; i += 256) {
-      const int x = i / a.cin, c = i % a.cin;
+    for (int i = threadIdx.x; i < W * (a.cin / 8); i += 256) {
+      const int q = i / W, x = i - q * W;
       const int xs = x + dx;
-      xT[c * WP + x] = (zy && xs >= 0 && xs < W) ? xrow[(long long)xs * a.cin + c] : (u16)0;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (zy && xs >= 0 && xs < W)
+        v = *reinterpret_cast<const uint4*>(xrow + (long long)xs * a.cin + 8 * q);
+      const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        xT[(8 * q + 2 * e) * WP + x] = (u16)(u[e] & 0xffffu);
+        xT[(8 * q + 2 * e + 1) * WP + x] = (u16)(u[e] >> 16);
+      }
     }
     __syncthreads();
     for (int k0 = 0; k0 < W; k0 += 32) {
