@@ -11,6 +11,7 @@
 //   cudaCTForwardProjection_f32 genProj/cudaIterType.cuh:280 (core arguments only)
 //   ProjectionImageT::LoadFolder/SaveToFolder, VolumeT::Save/Load
 //                               common/type.h:1796-1828, 3831-3858, 3256-3310
+//   CheckImageQuality, NCC      common/type.h:4991-5030, 4960-4987
 //
 // Failures throw std::string, as the reference's VERIFY_TRUE / CUDA_CHECK_ERROR
 // do (common/type.h:194-196, common/cudaLib.cuh:49-56); the message is the
@@ -439,6 +440,63 @@ inline bool cudaCTForwardProjection_f32(const Parameter& para, const std::vector
                                             FDK_FP_CLAMP, out.buffer()),
                         "cudaCTForwardProjection_f32");
   return true;
+}
+
+// CheckImageQuality (common/type.h:4991-5030): error per voxel, MSE, PSNR
+// against a peak of 2.0, QI, and the pass thresholds (PSNR >= snrThreshold,
+// error per voxel <= errorPerPixelThreshold).  Prints the reference's report
+// to stdout and returns the figures; refLowerThreshold / refUpperThreshold are
+// accepted and unused, as in the reference.
+struct ImageQuality {
+  double error_per_pixel, mse, psnr, qi;
+  bool passed;
+};
+
+template <typename R, typename X>
+inline ImageQuality CheckImageQuality(const VolumeT<R>& ref, const VolumeT<X>& recon,
+                                      double refLowerThreshold = 0.0,
+                                      double refUpperThreshold = 0.0, double snrThreshold = 26.0,
+                                      double errorPerPixelThreshold = 0.03) {
+  (void)refLowerThreshold;
+  (void)refUpperThreshold;
+  if (ref.nx != recon.nx || ref.ny != recon.ny || ref.nz != recon.nz || ref.data.empty())
+    throw std::string("CheckImageQuality: volume sizes differ");
+  double abs_sum = 0.0, sq_sum = 0.0;
+  for (size_t i = 0; i < ref.data.size(); ++i) {
+    const double d = (double)recon.data[i] - (double)ref.data[i];
+    abs_sum += std::fabs(d);
+    sq_sum += d * d;
+  }
+  ImageQuality q;
+  q.error_per_pixel = abs_sum / ref.data.size();
+  q.mse = sq_sum / ref.data.size();
+  q.psnr = 20.0 * std::log10(2.0) - 10.0 * std::log10(q.mse);
+  q.qi = (2.0 - q.error_per_pixel) / 2.0;
+  q.passed = q.error_per_pixel <= errorPerPixelThreshold && q.psnr >= snrThreshold;
+  std::printf("\nError per Pixel = %g\nMSE = %g\nPSNR = %gdB\nQI = %g\n", q.error_per_pixel, q.mse,
+              q.psnr, q.qi);
+  return q;
+}
+
+// NCC (common/type.h:4960-4987): normalised cross-correlation about the means.
+template <typename A, typename B>
+inline double NCC(const A* a, const B* b, size_t n) {
+  if (n == 0) return 0.0;
+  double m0 = 0.0, m1 = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    m0 += a[i];
+    m1 += b[i];
+  }
+  m0 /= n;
+  m1 /= n;
+  double x = 0.0, y0 = 0.0, y1 = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double s0 = a[i] - m0, s1 = b[i] - m1;
+    x += s0 * s1;
+    y0 += s0 * s0;
+    y1 += s1 * s1;
+  }
+  return x / (std::sqrt(y0) * std::sqrt(y1));
 }
 
 #endif  // FDK_AMD_HPP

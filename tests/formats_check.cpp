@@ -26,5 +26,11 @@ int main(int argc, char** argv) {
   if (!w.Save(dir + "vol_cpp.raw")) return 6;
   if (!w.SaveSlices(dir + "slices_cpp/", 10)) return 7;
   if (!p.SaveToFolder(dir + "proj_cpp/", 3)) return 8;
+  VolumeT<float> shifted = w;
+  for (float& x : shifted.data) x += 0.05f;
+  const ImageQuality q = CheckImageQuality(w, shifted, -1024, 1024);
+  std::printf("quality %.9f %.9f %.9f %.9f %d\n", q.error_per_pixel, q.mse, q.psnr, q.qi,
+              (int)q.passed);
+  std::printf("ncc %.12f\n", NCC(w.data.data(), shifted.data.data(), w.data.size()));
   return 0;
 }

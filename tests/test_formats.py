@@ -70,7 +70,7 @@ def test_cpp_header_reads_and_writes_the_same_files(tmp_path):
     fm.save_volume(str(tmp_path / "vol.raw"), vol)
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
-    lines = dict((ln.split()[0], ln.split()[1:]) for ln in r.stdout.splitlines())
+    lines = dict((ln.split()[0], ln.split()[1:]) for ln in r.stdout.splitlines() if ln.strip())
     loaded = sum(float(views[k].sum()) for k in (0, 3, 6))
     assert float(lines["proj_sum"][0]) == pytest.approx(loaded, rel=1e-9)
     assert float(lines["proj_sum"][2]) == views[3, 0, 0]
@@ -83,3 +83,11 @@ def test_cpp_header_reads_and_writes_the_same_files(tmp_path):
     assert np.array_equal(s, w[1].ravel())
     back = fm.load_projection_folder(str(tmp_path / "proj_cpp"), 6, 5, 7, step=3)
     assert np.array_equal(back[3], views[3]) and np.array_equal(back[6], views[6])
+    # the C++ CheckImageQuality / NCC agree with the Python restatement
+    from fdk_amd.quality import check_image_quality, ncc
+    wv = 0.5 * np.arange(24, dtype=np.float32)
+    q = check_image_quality(wv, wv + np.float32(0.05))
+    got = [float(t) for t in lines["quality"][:4]]
+    assert got == pytest.approx([q["error_per_pixel"], q["mse"], q["psnr_db"], q["qi"]], rel=1e-6)
+    assert int(lines["quality"][4]) == int(q["passed"])
+    assert float(lines["ncc"][0]) == pytest.approx(ncc(wv, wv + np.float32(0.05)), abs=1e-9)

@@ -78,6 +78,12 @@ def test_c1_fdk_matches_oracle_and_phantom(torch_cuda):
     print(f"C1 vs phantom (central 64^3): least-squares scale {scale:.3f}, correlation {corr:.3f}")
     assert 0.9 < scale < 1.1
     assert corr > 0.9
+    # the reference's own report after FP -> BP (3dct/3dct.cpp:1424, CheckImageQuality)
+    from fdk_amd.quality import check_image_quality, ncc
+    q = check_image_quality(ph, vol)
+    print(f"C1 vs phantom (whole volume): error/voxel {q['error_per_pixel']:.4f}, "
+          f"PSNR {q['psnr_db']:.1f} dB, QI {q['qi']:.4f}, NCC {ncc(ph, vol):.4f}")
+    assert q["psnr_db"] > 26.0 and q["error_per_pixel"] < 0.03  # the reference's thresholds
 
 
 def c2(views):
