@@ -8,7 +8,9 @@ reference's BackProjectionFDK::BP batch).  Projections are exact line integrals
 of a seeded 40-ellipsoid phantom, cos-weighted and ramp/Hann filtered on the
 GPU before the timed region; the timed region holds only backprojection passes.
 
-Prints one JSON line (see DESIGN.md for every field's definition).
+Prints one JSON line: the backprojection rate and its roofline, the filter and
+forward-projection stages, the reference CPU baseline and the learned
+sparse-view comparison.
 """
 from __future__ import annotations
 
@@ -311,8 +313,8 @@ def main():
         stream_main(args)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here "
-              "has one MI355X; the multi-GPU sharding design is described in DESIGN.md)",
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path runs "
+              "on a single MI355X; several GPUs would each take a z-slab pair of the volume)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
