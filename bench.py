@@ -144,7 +144,7 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
                                     "sources" if use_ref else "oracle port (FP64)"))}
 
 
-def fp_stage_c1():
+def fp_stage_256cube():
     """Ray-marching forward projection at the 256^3 / 360-view geometry (1 mm
     voxels, 256^2 detector of 1.6 mm, SOD 1000, SDD 1500) with a 0.5 mm step,
     clamp-to-edge: samples/s, timed with events over 3 repetitions.  The sample
@@ -182,15 +182,15 @@ def fp_stage_c1():
             "bytes_per_sample": 32}
 
 
-def learned_c4():
+def learned_sparse_view():
     """The learned sparse-view comparison on a held-out phantom: FDK from all
     1024 noisy views of a 1024^3 scan against FDK from every 4th view followed
-    by the trained U-Net (fdk_amd/weights/unet_c4.npz), with GPU wall times
+    by the trained U-Net (fdk_amd/weights/unet_sparse1024.npz), with GPU wall times
     (projections already in HBM) and RMSE / SSIM against the 1024-view FDK and
     against the phantom."""
     from fdk_amd import learned as Lr
     from fdk_amd.unet import UNet3D
-    w = os.path.join(HERE, "fdk_amd", "weights", "unet_c4.npz")
+    w = os.path.join(HERE, "fdk_amd", "weights", "unet_sparse1024.npz")
     if not os.path.exists(w):
         return None
     p = Lr.geometry(1024, 1024)
@@ -198,7 +198,7 @@ def learned_c4():
     net = UNet3D(base=32, seed=0)
     net.load(w)
     r = Lr.compare(net, sc, 100000)
-    r["weights"] = "fdk_amd/weights/unet_c4.npz"
+    r["weights"] = "fdk_amd/weights/unet_sparse1024.npz"
     r["phantom_seed"] = 100000
     return r
 
@@ -377,7 +377,7 @@ def main():
                     "bytes_per_view": p.nu * p.nv * (4 + s_out)}
     log(f"filter stage: {filter_stage}")
     del fout
-    fp_stage = fp_stage_c1()
+    fp_stage = fp_stage_256cube()
     log(f"forward-projection stage: {fp_stage}")
     mats = p.proj_matrices().astype(np.float32)
 
@@ -464,7 +464,7 @@ def main():
         del store, raw, vol
         torch.cuda.empty_cache()
         try:
-            learned = learned_c4()
+            learned = learned_sparse_view()
         except Exception as e:  # reported, never decides the backprojection result
             log(f"learned comparison failed: {e}")
             learned = {"error": str(e)}

@@ -1,6 +1,6 @@
 """Train the sparse-view U-Net on the 1024^3 / 1024-view scan and evaluate it.
 
-    python -m fdk_amd.train_c4 --phantoms 200 --steps 600 --out fdk_amd/weights/unet_c4.npz
+    python -m fdk_amd.train_sparse --phantoms 200 --steps 600 --out fdk_amd/weights/unet_sparse1024.npz
 
 Pairs: FDK from every 4th of 1024 noisy views (I0 = 1e5) and FDK from all 1024,
 over random 256^3 boxes of seeded ellipsoid phantoms, cut into 128^3 patches.
@@ -38,7 +38,7 @@ def main(argv=None):
     from .unet import UNet3D
 
     def log(m):
-        print(f"[train_c4] {m}", flush=True)
+        print(f"[train_sparse] {m}", flush=True)
 
     p = Lr.geometry(a.n, a.views)
     sc = Lr.Scanner(p)

@@ -19,7 +19,7 @@ import oracle as O  # noqa: E402
 
 TINY = dict(nx=16, ny=16, nz=16, nu=24, nv=20, du=1.6, dv=1.6, nproj=8,
             volume_position_angle_rad=-0.3, filter_type=5)
-C1 = dict(nx=256, ny=256, nz=256, dx=1.0, dy=1.0, dz=1.0, nu=256, nv=256, du=1.6, dv=1.6,
+GEOM_256 = dict(nx=256, ny=256, nz=256, dx=1.0, dy=1.0, dz=1.0, nu=256, nv=256, du=1.6, dv=1.6,
           nproj=360, volume_position_angle_rad=0.0, filter_type=5)
 ODD = dict(nx=20, ny=18, nz=14, dx=1.3, dy=1.1, dz=0.9, nu=30, nv=26, du=1.2, dv=1.4,
            offset_nu=1.5, offset_nv=-2.0, nproj=12, scan_angle_deg=200.0, direction=1,
@@ -37,7 +37,7 @@ def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libfdkref.so is missing (run oracle/build_ref.sh)")
     rng = np.random.default_rng(2024)
-    for name, geom in (("tables_tiny.npz", TINY), ("tables_c1.npz", C1), ("tables_odd.npz", ODD)):
+    for name, geom in (("tables_tiny.npz", TINY), ("tables_256.npz", GEOM_256), ("tables_odd.npz", ODD)):
         g = O.Geom(**geom)
         r = O.ref_tables(g)
         save(name, geom, cos=r["cos"], filter=r["filter"], mats=r["mats"], af=np.array(r["af"]))

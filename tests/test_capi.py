@@ -62,7 +62,7 @@ def test_geometry_tables_match_oracle(kw):
 
 
 def test_geometry_matches_reference_fixture():
-    z = np.load(os.path.join(ROOT, "tests", "golden", "tables_c1.npz"), allow_pickle=False)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "tables_256.npz"), allow_pickle=False)
     import json
     p = F.Parameter(**json.loads(str(z["geom"])))
     assert np.array_equal(p.cos_table(), z["cos"])

@@ -21,7 +21,7 @@ def rel_rmse(a, b):
     return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
 
 
-def c1():
+def geom_256():
     return F.Parameter(nx=256, ny=256, nz=256, dx=1.0, dy=1.0, dz=1.0, nu=256, nv=256, du=1.6,
                        dv=1.6, sod=1000.0, sid=1500.0, nproj=360, volume_position_angle_rad=0.0,
                        filter_type=5)
@@ -51,7 +51,7 @@ def gpu_fdk(torch, p, raw, pad=F.PAD_MEAN4, dtype=F.FDK_F32):
 
 def test_c1_fdk_matches_oracle_and_phantom(torch_cuda):
     torch = torch_cuda
-    p = c1()
+    p = geom_256()
     ell = F.random_ellipsoids(10, seed=1, radius_mm=120.0, half_height_mm=120.0)
     phantom = torch.empty((p.nz, p.ny, p.nx), device="cuda")
     F.phantom_rasterize(ell, p, phantom)
@@ -66,7 +66,7 @@ def test_c1_fdk_matches_oracle_and_phantom(torch_cuda):
     filt = O.filter_rows(O.cos_weight(proj, p.cos_table()), H, O.PAD_MEAN4)
     ref = O.bp(g, filt, p.proj_matrices().astype(np.float32).astype(np.float64), O.BP_GPU_STRICT)
     err = rel_rmse(vol, ref)
-    print(f"C1 256^3 x 360: relative RMSE vs oracle FDK {err:.2e}")
+    print(f"256^3 x 360 views: relative RMSE vs oracle FDK {err:.2e}")
     assert err < 1e-4
 
     # the reconstruction is the phantom up to FDK's cone-beam and sampling error:
@@ -75,18 +75,18 @@ def test_c1_fdk_matches_oracle_and_phantom(torch_cuda):
     core = (slice(96, 160),) * 3
     scale = float(np.sum(vol[core] * ph[core]) / np.sum(ph[core] ** 2))
     corr = float(np.corrcoef(vol[core].ravel(), ph[core].ravel())[0, 1])
-    print(f"C1 vs phantom (central 64^3): least-squares scale {scale:.3f}, correlation {corr:.3f}")
+    print(f"256^3 vs phantom (central 64^3): least-squares scale {scale:.3f}, correlation {corr:.3f}")
     assert 0.9 < scale < 1.1
     assert corr > 0.9
     # the reference's own report after FP -> BP (3dct/3dct.cpp:1424, CheckImageQuality)
     from fdk_amd.quality import check_image_quality, ncc
     q = check_image_quality(ph, vol)
-    print(f"C1 vs phantom (whole volume): error/voxel {q['error_per_pixel']:.4f}, "
+    print(f"256^3 vs phantom (whole volume): error/voxel {q['error_per_pixel']:.4f}, "
           f"PSNR {q['psnr_db']:.1f} dB, QI {q['qi']:.4f}, NCC {ncc(ph, vol):.4f}")
     assert q["psnr_db"] > 26.0 and q["error_per_pixel"] < 0.03  # the reference's thresholds
 
 
-def c2(views):
+def geom_2048(views):
     return F.Parameter(nx=2048, ny=2048, nz=2048, dx=0.2 / 1.5, dy=0.2 / 1.5, dz=0.2 / 1.5,
                        nu=2048, nv=2048, du=0.2, dv=0.2, sod=1000.0, sid=1500.0, nproj=4096,
                        volume_position_angle_rad=0.0, filter_type=5)
@@ -94,7 +94,7 @@ def c2(views):
 
 def test_c2_full_size_sampled_voxels_and_batch_split(torch_cuda):
     torch = torch_cuda
-    p = c2(4096)
+    p = geom_2048(4096)
     n = 2048
     V = 64
     ell = F.random_ellipsoids(40, seed=2, radius_mm=120.0, half_height_mm=120.0, axes_mm=(5, 100))
@@ -125,7 +125,7 @@ def test_c2_full_size_sampled_voxels_and_batch_split(torch_cuda):
     views = np.ascontiguousarray(store.transpose(1, 2).cpu().numpy())  # (V, nv, nu)
     want = O.bp_voxels(g, views, mats.astype(np.float64), ijk, O.BP_GPU_STRICT)
     err = rel_rmse(got, want)
-    print(f"C2 2048^3 x {V} views, {len(ijk)} sampled voxels: relative RMSE vs oracle {err:.2e}")
+    print(f"2048^3 x {V} views, {len(ijk)} sampled voxels: relative RMSE vs oracle {err:.2e}")
     assert err < 1e-5
 
     # batch-split invariance: the same 32 views in two 16-view passes

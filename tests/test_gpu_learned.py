@@ -1,5 +1,5 @@
-"""The trained sparse-view network (fdk_amd/weights/unet_c4.npz, trained by
-``python -m fdk_amd.train_c4`` on seeds 0..159) on a held-out phantom: FDK from
+"""The trained sparse-view network (fdk_amd/weights/unet_sparse1024.npz, trained by
+``python -m fdk_amd.train_sparse`` on seeds 0..159) on a held-out phantom: FDK from
 256 of 1024 noisy views plus the network must come closer to the 1024-view FDK
 than the 256-view FDK alone, on a central 256^3 box of the 1024^3 scan, with
 128^3 tiles and with whole-slice z-slabs."""
@@ -11,7 +11,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fdk_amd",
-                       "weights", "unet_c4.npz")
+                       "weights", "unet_sparse1024.npz")
 
 
 def test_trained_network_improves_sparse_fdk_on_held_out_phantom(torch_cuda):

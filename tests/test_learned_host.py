@@ -18,7 +18,7 @@ def test_tiles_partition_the_axis(n):
         assert len(t) == 9                    # 9^3 = 729 tiles for the 1024^3 volume
 
 
-def test_c4_geometry():
+def test_sparse_view_geometry():
     p = Lr.geometry(1024, 1024)
     assert (p.nx, p.nu, p.nproj) == (1024, 1024, 1024)
     assert p.du == pytest.approx(0.4) and p.dx == pytest.approx(0.4 / 1.5)

@@ -25,7 +25,7 @@ def rel(a, b):
 
 
 # ------------------------------------------------------------ geometry tables
-@pytest.mark.parametrize("name", ["tables_tiny.npz", "tables_c1.npz", "tables_odd.npz"])
+@pytest.mark.parametrize("name", ["tables_tiny.npz", "tables_256.npz", "tables_odd.npz"])
 def test_tables_match_reference(name):
     z, geom = load(name)
     g = O.Geom(**geom)
