@@ -679,6 +679,157 @@ __global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w,
   if (threadIdx.x == 0) atomicAdd(db, red[c]);
 }
 
+// Space-to-depth at the network's ends.  An FP32 volume [n][2d][2h][2w] maps to
+// the half-resolution grid [n][d][h][w][cy] BF16: voxel (z, y, x) holds the
+// 2x2x2 block (2z + iz, 2y + iy, 2x + ix) as channels 4 iz + 2 iy + ix, and
+// channels 8..cy-1 are zero.  One thread per half-resolution voxel; the two
+// x-neighbours of a block row are one 8-byte load.
+__device__ inline unsigned pack_bf2(float lo, float hi) {
+  return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+}
+
+__global__ __launch_bounds__(256) void unshuffle2_kernel(const float* x, uint4* y, int h2, int w2,
+                                                         int cyq, long long nvox) {
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvox) return;
+  const int xx = (int)(v % w2);
+  const long long r = v / w2;  // (n d + z) h + y of the half-resolution voxel
+  const int yy = (int)(r % h2);
+  const long long nz = r / h2;
+  const long long W = 2LL * w2, H = 2LL * h2;
+  const float* base = x + ((2 * nz) * H + 2 * yy) * W + 2 * xx;
+  float2 q[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)  // k = 2 iz + iy
+    q[k] = *reinterpret_cast<const float2*>(base + ((k >> 1) * H + (k & 1)) * W);
+  uint4* o = y + v * cyq;
+  o[0] = make_uint4(pack_bf2(q[0].x, q[0].y), pack_bf2(q[1].x, q[1].y),
+                    pack_bf2(q[2].x, q[2].y), pack_bf2(q[3].x, q[3].y));
+  for (int j = 1; j < cyq; ++j) o[j] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Depth-to-space output head: for half-resolution voxel v, eight outputs
+// s_k = b[k] + sum_c w[k][c] h[v][c] land on its 2x2x2 block k = 4 iz + 2 iy + ix
+// of the full-resolution volume: out = in + s.  C == 32; w, b staged in LDS.
+template <int C>
+__global__ __launch_bounds__(256) void head_shuffle_kernel(const uint4* h, const float* w,
+                                                           const float* b, const float* in,
+                                                           float* out, int h2, int w2,
+                                                           long long nvox) {
+  __shared__ float ws[8 * C + 8];
+  for (int i = threadIdx.x; i < 8 * C; i += 256) ws[i] = w[i];
+  if (threadIdx.x < 8) ws[8 * C + threadIdx.x] = b[threadIdx.x];
+  __syncthreads();
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvox) return;
+  float hv[C];
+#pragma unroll
+  for (int q = 0; q < C / 8; ++q) {
+    const uint4 t = h[v * (C / 8) + q];
+    const unsigned u[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      hv[8 * q + 2 * e] = __uint_as_float(u[e] << 16);
+      hv[8 * q + 2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
+    }
+  }
+  float s[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float a = ws[8 * C + k];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a = fmaf(ws[k * C + c], hv[c], a);
+    s[k] = a;
+  }
+  const int xx = (int)(v % w2);
+  const long long r = v / w2;
+  const int yy = (int)(r % h2);
+  const long long nz = r / h2;
+  const long long W = 2LL * w2, H = 2LL * h2;
+  const long long o0 = ((2 * nz) * H + 2 * yy) * W + 2 * xx;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long long o = o0 + ((k >> 1) * H + (k & 1)) * W;
+    const float2 i2 = *reinterpret_cast<const float2*>(in + o);
+    *reinterpret_cast<float2*>(out + o) = make_float2(i2.x + s[2 * k], i2.y + s[2 * k + 1]);
+  }
+}
+
+// Adjoint of head_shuffle_kernel: dh[v][c] = sum_k w[k][c] g_k (BF16),
+// dw[k][c] += sum_v g_k h[v][c], db[k] += sum_v g_k, with g_k = dout at block k
+// of voxel v.  A block walks tiles of 256 voxels; each tile's g and h go through
+// LDS so thread (k, c) = (t / C, t % C) accumulates its dw entry.  C == 32.
+template <int C>
+__global__ __launch_bounds__(256) void head_shuffle_bwd_kernel(const float* dout, const uint4* h,
+                                                               const float* w, uint4* dh,
+                                                               float* dw, float* db, int h2,
+                                                               int w2, long long nvox) {
+  static_assert(8 * C == 256, "one dw entry per thread");
+  __shared__ float ws[8 * C];
+  __shared__ float gs[256][9];
+  __shared__ float hs[256][C + 1];
+  for (int i = threadIdx.x; i < 8 * C; i += 256) ws[i] = w[i];
+  __syncthreads();
+  const int t = threadIdx.x, kk = t / C, cc = t % C;
+  const long long W = 2LL * w2, H = 2LL * h2;
+  float acc = 0.f, accb = 0.f;
+  for (long long tile = (long long)blockIdx.x * 256; tile < nvox; tile += (long long)gridDim.x * 256) {
+    const long long v = tile + t;
+    float g[8], hv[C];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) g[k] = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) hv[c] = 0.f;
+    if (v < nvox) {
+      const int xx = (int)(v % w2);
+      const long long r = v / w2;
+      const int yy = (int)(r % h2);
+      const long long nz = r / h2;
+      const long long o0 = ((2 * nz) * H + 2 * yy) * W + 2 * xx;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float2 g2 = *reinterpret_cast<const float2*>(dout + o0 + ((k >> 1) * H + (k & 1)) * W);
+        g[2 * k] = g2.x;
+        g[2 * k + 1] = g2.y;
+      }
+#pragma unroll
+      for (int q = 0; q < C / 8; ++q) {
+        const uint4 u4 = h[v * (C / 8) + q];
+        const unsigned u[4] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          hv[8 * q + 2 * e] = __uint_as_float(u[e] << 16);
+          hv[8 * q + 2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < C / 8; ++q) {
+        float d8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          float a = 0.f;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) a = fmaf(ws[k * C + 8 * q + e], g[k], a);
+          d8[e] = a;
+        }
+        dh[v * (C / 8) + q] = make_uint4(pack_bf2(d8[0], d8[1]), pack_bf2(d8[2], d8[3]),
+                                         pack_bf2(d8[4], d8[5]), pack_bf2(d8[6], d8[7]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gs[t][k] = g[k];
+#pragma unroll
+    for (int c = 0; c < C; ++c) hs[t][c] = hv[c];
+    __syncthreads();
+    for (int j = 0; j < 256; ++j) acc = fmaf(gs[j][kk], hs[j][cc], acc);
+    if (t < 8)
+      for (int j = 0; j < 256; ++j) accb += gs[j][t];
+    __syncthreads();
+  }
+  atomicAdd(&dw[t], acc);
+  if (t < 8) atomicAdd(&db[t], accb);
+}
+
 // Mean-squared error: loss_sum += sum (out - t)^2, dout = 2 (out - t) / nvox.
 __global__ void mse_kernel(const float* out, const float* t, float* dout, float* loss_sum,
                            long long nvox) {
@@ -984,6 +1135,58 @@ extern "C" int fdk_nn_head_bwd(const float* dout, const void* h, const float* w,
   hipLaunchKernelGGL(head_bwd_kernel, dim3(std::min<long long>(1024, blocks_for(nvox))), dim3(256), 0,
                      static_cast<hipStream_t>(stream), dout, static_cast<const u16*>(h), w,
                      static_cast<u16*>(dh), dw, db, (long long)nvox, c);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_unshuffle2(const float* x, void* y, int n, int d, int h, int w, int cy,
+                                 void* stream) {
+  FDK_REQUIRE(x && y, "NULL pointer");
+  FDK_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0 && d % 2 == 0 && h % 2 == 0 && w % 2 == 0,
+              "unshuffle needs even, positive d, h, w (got %d, %d, %d)", d, h, w);
+  FDK_REQUIRE(cy >= 8 && cy % 8 == 0, "cy must be a multiple of 8, at least 8 (got %d)", cy);
+  FDK_REQUIRE(reinterpret_cast<uintptr_t>(x) % 8 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0,
+              "unshuffle needs 8-byte aligned x and 16-byte aligned y");
+  const long long nvox = (long long)n * (d / 2) * (h / 2) * (w / 2);
+  hipLaunchKernelGGL(unshuffle2_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), x, static_cast<uint4*>(y), h / 2, w / 2,
+                     cy / 8, nvox);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_head_shuffle(const void* h, const float* w, const float* b, const float* in,
+                                   float* out, int n, int d, int hh, int ww, int c, void* stream) {
+  FDK_REQUIRE(h && w && b && in && out, "NULL pointer");
+  FDK_REQUIRE(c == 32, "the depth-to-space head takes 32 channels (got %d)", c);
+  FDK_REQUIRE(n > 0 && d > 0 && hh > 0 && ww > 0 && d % 2 == 0 && hh % 2 == 0 && ww % 2 == 0,
+              "head_shuffle needs even, positive d, h, w (got %d, %d, %d)", d, hh, ww);
+  FDK_REQUIRE(reinterpret_cast<uintptr_t>(h) % 16 == 0 && reinterpret_cast<uintptr_t>(in) % 8 == 0 &&
+                  reinterpret_cast<uintptr_t>(out) % 8 == 0,
+              "head_shuffle needs 16-byte aligned h and 8-byte aligned in / out");
+  const long long nvox = (long long)n * (d / 2) * (hh / 2) * (ww / 2);
+  hipLaunchKernelGGL(head_shuffle_kernel<32>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const uint4*>(h), w, b, in, out,
+                     hh / 2, ww / 2, nvox);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_head_shuffle_bwd(const float* dout, const void* h, const float* w, void* dh,
+                                       float* dw, float* db, int n, int d, int hh, int ww, int c,
+                                       void* stream) {
+  FDK_REQUIRE(dout && h && w && dh && dw && db, "NULL pointer");
+  FDK_REQUIRE(c == 32, "the depth-to-space head takes 32 channels (got %d)", c);
+  FDK_REQUIRE(n > 0 && d > 0 && hh > 0 && ww > 0 && d % 2 == 0 && hh % 2 == 0 && ww % 2 == 0,
+              "head_shuffle needs even, positive d, h, w (got %d, %d, %d)", d, hh, ww);
+  FDK_REQUIRE(reinterpret_cast<uintptr_t>(h) % 16 == 0 && reinterpret_cast<uintptr_t>(dh) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(dout) % 8 == 0,
+              "head_shuffle_bwd needs 16-byte aligned h, dh and 8-byte aligned dout");
+  const long long nvox = (long long)n * (d / 2) * (hh / 2) * (ww / 2);
+  hipLaunchKernelGGL(head_shuffle_bwd_kernel<32>, dim3(std::min<long long>(1024, (nvox + 255) / 256)),
+                     dim3(256), 0, static_cast<hipStream_t>(stream), dout,
+                     static_cast<const uint4*>(h), w, static_cast<uint4*>(dh), dw, db, hh / 2,
+                     ww / 2, nvox);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

@@ -10,10 +10,13 @@ Pipeline pieces:
                       photon noise (I0 = 1e5), GPU FDK from all or every k-th view
                       into any sub-box.
 * ``make_pairs``      training pairs (FDK from 1/4 of the views, FDK from all)
-                      cut into 128^3 patches.
+                      cut into cubic patches (128^3; 256^3 for the
+                      space-to-depth network).
 * ``train``           Adam on batches of patches, all in the library's kernels.
-* ``infer_tiled``     128^3 tiles with a 16-voxel overlap over a whole volume;
+* ``infer_tiled``     cubic tiles with a 16-voxel overlap over a whole volume;
                       each tile contributes its centre, borders at the edges.
+* ``infer_slabs``     whole x-y slices in overlapping z-slabs, or the whole
+                      volume in one pass.
 """
 from __future__ import annotations
 
@@ -105,15 +108,16 @@ class Scanner:
         return vol
 
 
-def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4, rng=None):
+def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4, rng=None,
+               patch: int = PATCH):
     """For each seed: a noisy scan, FDK from every sparse_step-th view and from all
-    views inside a random region^3 box, cut into (region/128)^3 patches.
-    Returns (inputs, targets), each a (P, 128, 128, 128) FP32 device tensor."""
+    views inside a random region^3 box, cut into (region/patch)^3 patches.
+    Returns (inputs, targets), each a (P, patch, patch, patch) FP32 device tensor."""
     t = scanner.torch
     p = scanner.p
     rng = np.random.default_rng(0) if rng is None else rng
-    per = (region // PATCH) ** 3
-    xs = t.empty((len(seeds) * per, PATCH, PATCH, PATCH), device="cuda")
+    per = (region // patch) ** 3
+    xs = t.empty((len(seeds) * per, patch, patch, patch), device="cuda")
     ys = t.empty_like(xs)
     raw = t.empty((p.nproj, p.nv, p.nu), device="cuda")
     store = t.empty((p.nproj, p.nu, p.nv), device="cuda")
@@ -126,11 +130,11 @@ def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4,
         box = (org[0], org[1], org[2], region, region, region)
         sparse = scanner.fdk(raw, sparse_step, box, store)
         full = scanner.fdk(raw, 1, box, store)
-        for a in range(0, region, PATCH):
-            for b in range(0, region, PATCH):
-                for c in range(0, region, PATCH):
-                    box_copy(sparse, (a, b, c), xs[q], (0, 0, 0), (PATCH, PATCH, PATCH))
-                    box_copy(full, (a, b, c), ys[q], (0, 0, 0), (PATCH, PATCH, PATCH))
+        for a in range(0, region, patch):
+            for b in range(0, region, patch):
+                for c in range(0, region, patch):
+                    box_copy(sparse, (a, b, c), xs[q], (0, 0, 0), (patch, patch, patch))
+                    box_copy(full, (a, b, c), ys[q], (0, 0, 0), (patch, patch, patch))
                     q += 1
     t.cuda.synchronize()
     return xs, ys
@@ -166,7 +170,7 @@ def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, see
         losses.append(loss)
         if log and (s % 25 == 0 or s == steps - 1):
             log(f"step {s}: mse {loss.item() / x.numel():.4e} ({time.perf_counter() - t0:.1f} s)")
-    return [l.item() / (batch * PATCH ** 3) for l in losses]
+    return [l.item() / (batch * xs[0].numel()) for l in losses]
 
 
 def tiles(n: int, patch: int = PATCH, overlap: int = OVERLAP):
@@ -181,19 +185,25 @@ def tiles(n: int, patch: int = PATCH, overlap: int = OVERLAP):
     return [(s, cuts[k], cuts[k + 1]) for k, s in enumerate(starts)]
 
 
+def tile_size(net: UNet3D) -> int:
+    """The cubic tile the network takes: 128, or 256 for the space-to-depth form."""
+    return 2 * PATCH if net.shuffle else PATCH
+
+
 def infer_tiled(net: UNet3D, vol, out=None, batch: int = 4):
-    """Apply the network to a (D, H, W) FP32 device volume in overlapping
-    128^3 tiles; D, H, W >= 128."""
+    """Apply the network to a (D, H, W) FP32 device volume in overlapping cubic
+    tiles of tile_size(net); D, H, W >= that size."""
     import torch
     D, H, W = vol.shape
+    T = tile_size(net)
     out = torch.empty_like(vol) if out is None else out
-    tz, ty, tx = tiles(D), tiles(H), tiles(W)
+    tz, ty, tx = tiles(D, T), tiles(H, T), tiles(W, T)
     jobs = [(a, b, c) for a in tz for b in ty for c in tx]
-    buf = torch.empty((batch, PATCH, PATCH, PATCH), device=vol.device)
+    buf = torch.empty((min(batch, len(jobs)), T, T, T), device=vol.device)
     for i in range(0, len(jobs), batch):
         part = jobs[i:i + batch]
         for j, ((sz, _, _), (sy, _, _), (sx, _, _)) in enumerate(part):
-            box_copy(vol, (sz, sy, sx), buf[j], (0, 0, 0), (PATCH, PATCH, PATCH))
+            box_copy(vol, (sz, sy, sx), buf[j], (0, 0, 0), (T, T, T))
         res = net.forward(buf[:len(part)])
         for j, ((sz, z0, z1), (sy, y0, y1), (sx, x0, x1)) in enumerate(part):
             box_copy(res[j], (z0 - sz, y0 - sy, x0 - sx), out, (z0, y0, x0),
@@ -201,14 +211,21 @@ def infer_tiled(net: UNet3D, vol, out=None, batch: int = 4):
     return out
 
 
-def infer_slabs(net: UNet3D, vol, out=None, slab: int = PATCH, overlap: int = OVERLAP):
+def infer_slabs(net: UNet3D, vol, out=None, slab: int | None = None, overlap: int = OVERLAP):
     """Apply the network to a (D, H, W) FP32 device volume in z-slabs of whole
     x-y slices (slab deep, neighbouring slabs overlapping by ``overlap`` and
-    split at the midpoint), batch 1: 1.125x redundant work at 1024^3 against
-    1.43x for 128^3 tiles.  D >= slab; H, W as UNet3D requires."""
+    split at the midpoint), batch 1: 1.125x redundant work at 1024^3 with
+    128-deep slabs against 1.43x for 128^3 tiles.  slab=None: 128 for the
+    full-resolution network, the whole volume in one pass (no redundant work)
+    for the space-to-depth network.  D >= slab; H, W as UNet3D requires."""
     import torch
     D = vol.shape[0]
+    if slab is None:
+        slab = D if net.shuffle else PATCH
     out = torch.empty_like(vol) if out is None else out
+    if slab >= D:
+        net.forward(vol.unsqueeze(0), out=out.unsqueeze(0))
+        return out
     for sz, z0, z1 in tiles(D, slab, overlap):
         res = net.forward(vol[sz:sz + slab].unsqueeze(0))
         out[z0:z1].copy_(res[0, z0 - sz:z1 - sz])
@@ -253,7 +270,7 @@ def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=
     scanner.tables(p.nproj // sparse_step)
     # warm the kernels once on a small box
     scanner.fdk(raw, 1, (0, 0, 0, 64, 64, 64), store)
-    net.forward(torch.zeros((1, PATCH, PATCH, PATCH), device="cuda"))
+    net.forward(torch.zeros((1,) + tuple(net.min_shape), device="cuda"))
     torch.cuda.synchronize()
 
     t0 = time.perf_counter()
@@ -265,7 +282,7 @@ def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=
     sparse = scanner.fdk(raw, sparse_step, None, store)
     torch.cuda.synchronize()
     t_sparse = time.perf_counter() - t0
-    net.forward(sparse[:PATCH].unsqueeze(0))  # warm the whole-slice shapes once
+    infer_slabs(net, sparse)  # warm: code objects and the allocator's blocks for these shapes
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     learned = infer_slabs(net, sparse)

@@ -1,11 +1,14 @@
 """Train the sparse-view U-Net on the 1024^3 / 1024-view scan and evaluate it.
 
     python -m fdk_amd.train_sparse --phantoms 200 --steps 600 --out fdk_amd/weights/unet_sparse1024.npz
+    python -m fdk_amd.train_sparse --shuffle --phantoms 480 --max-seconds 900 \
+        --out fdk_amd/weights/unet_sparse1024_s2d.npz
 
 Pairs: FDK from every 4th of 1024 noisy views (I0 = 1e5) and FDK from all 1024,
-over random 256^3 boxes of seeded ellipsoid phantoms, cut into 128^3 patches.
-Adam (lr 1e-3), batch 4.  The held-out comparison (``--eval-seeds``) runs on the
-whole 1024^3 volume.  Everything runs on the GPU in the library's kernels.
+over random 256^3 boxes of seeded ellipsoid phantoms, cut into 128^3 patches
+(whole 256^3 boxes for the space-to-depth network, ``--shuffle``).  Adam (lr
+1e-3), batch 4.  The held-out comparison (``--eval-seeds``) runs on the whole
+1024^3 volume.  Everything runs on the GPU in the library's kernels.
 """
 from __future__ import annotations
 
@@ -28,6 +31,10 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--max-seconds", type=float, default=None, help="training time budget")
     ap.add_argument("--init", default=None, help="start from saved weights")
+    ap.add_argument("--shuffle", action="store_true",
+                    help="space-to-depth network (runs on the half-resolution grid)")
+    ap.add_argument("--patch", type=int, default=None,
+                    help="training patch edge (default 128, 256 with --shuffle)")
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[100000])
     ap.add_argument("--report", default=None, help="write the evaluation as JSON")
     a = ap.parse_args(argv)
@@ -42,14 +49,18 @@ def main(argv=None):
 
     p = Lr.geometry(a.n, a.views)
     sc = Lr.Scanner(p)
-    net = UNet3D(base=32, seed=0)
+    patch = a.patch or (256 if a.shuffle else Lr.PATCH)
     if a.init:
-        net.load(a.init)
+        net = UNet3D.from_file(a.init)
+        if net.shuffle != a.shuffle:
+            raise SystemExit(f"{a.init} holds shuffle={net.shuffle} weights")
+    else:
+        net = UNet3D(base=32, seed=0, shuffle=a.shuffle)
     report = {"geometry": {"n": a.n, "views": a.views, "voxel_mm": p.dx, "pixel_mm": p.du}}
     if a.steps > 0:
         t0 = time.perf_counter()
-        xs, ys = Lr.make_pairs(sc, list(range(a.phantoms)), region=a.region,
-                               rng=np.random.default_rng(1))
+        xs, ys = Lr.make_pairs(sc, list(range(a.phantoms)), region=max(a.region, patch),
+                               rng=np.random.default_rng(1), patch=patch)
         log(f"{xs.shape[0]} patch pairs from {a.phantoms} phantoms in {time.perf_counter() - t0:.1f} s")
         t0 = time.perf_counter()
         losses = Lr.train(net, xs, ys, a.steps, a.batch, a.lr, seed=2, log=log,
@@ -57,7 +68,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         log(f"{len(losses)} steps in {dt:.1f} s ({dt / len(losses) * 1e3:.1f} ms/step)")
-        report["training"] = {"pairs": int(xs.shape[0]), "steps": len(losses), "batch": a.batch,
+        report["training"] = {"pairs": int(xs.shape[0]), "patch": patch, "shuffle": a.shuffle,
+                              "steps": len(losses), "batch": a.batch,
                               "lr": a.lr, "seconds": dt, "mse_first": float(np.mean(losses[:10])),
                               "mse_last": float(np.mean(losses[-10:]))}
         del xs, ys

@@ -4,6 +4,12 @@ Architecture: 3 levels, base 32 channels, two 3x3x3 conv + ReLU per level,
 2x2x2 average pooling down, nearest 2x upsampling up with channel-concatenated
 skips, a 1x1 output head, and a residual connection (output = input + net).
 
+With ``shuffle=True`` the network runs on the half-resolution grid: the input's
+2x2x2 voxel blocks become 8 channels (space-to-depth, zero-padded to 32 for the
+first convolution) and an 8-output head writes each block back
+(depth-to-space).  Nothing of the input is lost, and every convolution runs on
+1/8 of the voxels.
+
 Every operation runs in the library's HIP kernels (fdk_nn_*): BF16 MFMA
 convolutions with FP32 accumulation, BF16 channels-last activations, FP32
 master weights and Adam.  This module only sequences them; torch tensors serve
@@ -50,6 +56,9 @@ def _sig():
     L.fdk_nn_add.argtypes = [vp, vp, vp, i64, vp]
     L.fdk_nn_head.argtypes = [vp, vp, vp, vp, vp, i64, i, vp]
     L.fdk_nn_head_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i64, i, vp]
+    L.fdk_nn_unshuffle2.argtypes = [vp, vp, i, i, i, i, i, vp]
+    L.fdk_nn_head_shuffle.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, vp]
+    L.fdk_nn_head_shuffle_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, i, i, vp]
     L.fdk_nn_mse.argtypes = [vp, vp, vp, vp, i64, vp]
     L.fdk_nn_adam.argtypes = [vp, vp, vp, vp, vp, i64, f, f, f, f, i, i, i, vp]
     L.fdk_nn_convert.argtypes = [vp, vp, i64, i, vp]
@@ -117,6 +126,27 @@ def add(a, b, y):
     return y
 
 
+def unshuffle2(x32, y, cy=32):
+    """FP32 (N, D, H, W) -> BF16 (N, D/2, H/2, W/2, cy) space-to-depth."""
+    n, d, h, w = x32.shape
+    _check(L().fdk_nn_unshuffle2(_p(x32), _p(y), n, d, h, w, cy, _st()), "unshuffle2")
+    return y
+
+
+def head_shuffle(hid, w, b, x32, out):
+    n, d, h, wd = x32.shape
+    _check(L().fdk_nn_head_shuffle(_p(hid), _p(w), _p(b), _p(x32), _p(out), n, d, h, wd,
+                                   hid.shape[-1], _st()), "head_shuffle")
+    return out
+
+
+def head_shuffle_bwd(dout, hid, w, dh, dw, db):
+    n, d, h, wd = dout.shape
+    _check(L().fdk_nn_head_shuffle_bwd(_p(dout), _p(hid), _p(w), _p(dh), _p(dw), _p(db), n, d, h,
+                                       wd, hid.shape[-1], _st()), "head_shuffle_bwd")
+    return dh
+
+
 def to_bf16(src, dst):
     _check(L().fdk_nn_convert(_p(src), _p(dst), src.numel(), 1, _st()), "to_bf16")
     return dst
@@ -150,15 +180,19 @@ def poisson_noise(proj, i0, mu, seed):
 class Conv3:
     """3x3x3 'same' convolution (+ bias, optional ReLU) with Adam state."""
 
-    def __init__(self, cin, cout, relu, rng, dev):
+    def __init__(self, cin, cout, relu, rng, dev, live_cin=None):
+        """live_cin: input channels that carry data (the rest are zero padding
+        and get zero weights)."""
         torch = _torch()
         self.cin, self.cout, self.relu = cin, cout, relu
-        std = np.sqrt(2.0 / (27 * cin))
+        live = cin if live_cin is None else live_cin
+        std = np.sqrt(2.0 / (27 * live))
         if cin == 1:
             w = np.zeros((cout, 32), np.float32)
             w[:, :27] = rng.standard_normal((cout, 27)) * std
         else:
             w = (rng.standard_normal((27, cout, cin)) * std).astype(np.float32)
+            w[:, :, live:] = 0.0
         self.w = torch.as_tensor(w).to(dev)
         self.b = torch.zeros(cout, device=dev)
         self.wbf = torch.empty(self.w.shape, dtype=I16, device=dev)
@@ -204,14 +238,20 @@ def adam(w, wbf, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
 
 class UNet3D:
     """Residual U-Net: out = x + head(dec1(...)).  Input / output (N, D, H, W)
-    FP32 tensors; D, H, W multiples of 16 and W a multiple of 128."""
+    FP32 tensors; D % 4, H % 16 and W % 128 == 0 (shuffle:
+    D % 8, H % 32 and W % 256 == 0)."""
 
-    def __init__(self, base: int = 32, seed: int = 0, device="cuda"):
+    def __init__(self, base: int = 32, seed: int = 0, device="cuda", shuffle: bool = False):
         torch = _torch()
         rng = np.random.default_rng(seed)
         c1, c2, c3 = base, 2 * base, 4 * base
         self.c = (c1, c2, c3)
-        self.e1a = Conv3(1, c1, True, rng, device)
+        self.shuffle = bool(shuffle)
+        if self.shuffle:
+            assert c1 == 32, "the depth-to-space head takes 32 channels"
+            self.e1a = Conv3(32, c1, True, rng, device, live_cin=8)
+        else:
+            self.e1a = Conv3(1, c1, True, rng, device)
         self.e1b = Conv3(c1, c1, True, rng, device)
         self.e2a = Conv3(c1, c2, True, rng, device)
         self.e2b = Conv3(c2, c2, True, rng, device)
@@ -221,8 +261,10 @@ class UNet3D:
         self.d2b = Conv3(c2, c2, True, rng, device)
         self.d1a = Conv3(c2 + c1, c1, True, rng, device)
         self.d1b = Conv3(c1, c1, True, rng, device)
-        self.hw = torch.as_tensor((rng.standard_normal(c1) * 1e-3).astype(np.float32)).to(device)
-        self.hb = torch.zeros(1, device=device)
+        nh = 8 if self.shuffle else 1
+        hw = (rng.standard_normal((nh, c1)) * 1e-3).astype(np.float32)
+        self.hw = torch.as_tensor(hw if self.shuffle else hw[0]).to(device)
+        self.hb = torch.zeros(nh, device=device)
         self.ghw = torch.zeros_like(self.hw)
         self.ghb = torch.zeros_like(self.hb)
         self.hstate = [torch.zeros_like(self.hw), torch.zeros_like(self.hw),
@@ -235,38 +277,83 @@ class UNet3D:
         return [self.e1a, self.e1b, self.e2a, self.e2b, self.ba, self.bb, self.d2a, self.d2b,
                 self.d1a, self.d1b]
 
-    def forward(self, x32, keep=False):
+    def forward(self, x32, keep=False, out=None):
+        """x32 (N, D, H, W) FP32 -> (N, D, H, W) FP32, into ``out`` when given.
+        keep: hold every activation for backward().  Otherwise each is released
+        once its last reader is queued (the stream orders the reuse), so the
+        peak is about the decoder's concatenation plus two 32-channel tensors."""
         torch = _torch()
         c1, c2, c3 = self.c
         N, D, H, W = x32.shape
-        s1_, s2_, s3_ = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
+        self.check_shape(x32.shape)
         dev = x32.device
-        xb = to_bf16(x32, torch.empty(x32.shape, dtype=I16, device=dev))
+        k = {}
+
+        def bf(shape, c):
+            return torch.empty(shape + (c,), dtype=I16, device=dev)
+
+        def hold(**t):
+            if keep:
+                k.update(t)
+
+        if self.shuffle:
+            D, H, W = D // 2, H // 2, W // 2
+            xb = unshuffle2(x32, bf((N, D, H, W), 32))
+        else:
+            xb = to_bf16(x32, torch.empty(x32.shape, dtype=I16, device=dev))
+        s1_, s2_, s3_ = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
         e1a = self.e1a.forward(xb, s1_)
+        hold(xb=xb)
+        del xb
         s1 = self.e1b.forward(e1a, s1_)
-        p1 = avgpool(s1, torch.empty(s2_ + (c1,), dtype=I16, device=dev), s1_, c1)
+        hold(e1a=e1a)
+        del e1a
+        p1 = avgpool(s1, bf(s2_, c1), s1_, c1)
         e2a = self.e2a.forward(p1, s2_)
+        hold(p1=p1)
+        del p1
         s2 = self.e2b.forward(e2a, s2_)
-        p2 = avgpool(s2, torch.empty(s3_ + (c2,), dtype=I16, device=dev), s2_, c2)
+        hold(e2a=e2a)
+        del e2a
+        p2 = avgpool(s2, bf(s3_, c2), s2_, c2)
         ba = self.ba.forward(p2, s3_)
+        hold(p2=p2)
+        del p2
         bb = self.bb.forward(ba, s3_)
-        cat2 = torch.empty(s2_ + (c3 + c2,), dtype=I16, device=dev)
+        hold(ba=ba)
+        del ba
+        cat2 = bf(s2_, c3 + c2)
         upsample(bb, cat2, s3_, c3, c3 + c2, 0)
         channel_copy(s2, c2, 0, cat2, c3 + c2, c3, c2, int(np.prod(s2_)))
+        hold(bb=bb, s2=s2)
+        del bb, s2
         d2a = self.d2a.forward(cat2, s2_)
+        hold(cat2=cat2)
+        del cat2
         d2 = self.d2b.forward(d2a, s2_)
-        cat1 = torch.empty(s1_ + (c2 + c1,), dtype=I16, device=dev)
+        hold(d2a=d2a)
+        del d2a
+        cat1 = bf(s1_, c2 + c1)
         upsample(d2, cat1, s2_, c2, c2 + c1, 0)
         channel_copy(s1, c1, 0, cat1, c2 + c1, c2, c1, int(np.prod(s1_)))
+        hold(d2=d2, s1=s1)
+        del d2, s1
         d1a = self.d1a.forward(cat1, s1_)
+        hold(cat1=cat1)
+        del cat1
         d1 = self.d1b.forward(d1a, s1_)
-        out = torch.empty(x32.shape, dtype=torch.float32, device=dev)
-        _check(L().fdk_nn_head(_p(d1), _p(self.hw), _p(self.hb), _p(x32), _p(out),
-                               out.numel(), c1, _st()), "head")
+        hold(d1a=d1a)
+        del d1a
+        if out is None:
+            out = torch.empty(x32.shape, dtype=torch.float32, device=dev)
+        if self.shuffle:
+            head_shuffle(d1, self.hw, self.hb, x32, out)
+        else:
+            _check(L().fdk_nn_head(_p(d1), _p(self.hw), _p(self.hb), _p(x32), _p(out),
+                                   out.numel(), c1, _st()), "head")
         if keep:
-            self.cache = dict(shapes=(s1_, s2_, s3_), xb=xb, e1a=e1a, s1=s1, p1=p1, e2a=e2a, s2=s2,
-                              p2=p2, ba=ba, bb=bb, cat2=cat2, d2a=d2a, d2=d2, cat1=cat1, d1a=d1a,
-                              d1=d1)
+            k.update(d1=d1)
+            self.cache = dict(shapes=(s1_, s2_, s3_), **k)
         return out
 
     def zero_grad(self):
@@ -284,8 +371,11 @@ class UNet3D:
         dev = dout.device
         n1, n2 = int(np.prod(s1_)), int(np.prod(s2_))
         dd1 = torch.empty(s1_ + (c1,), dtype=I16, device=dev)
-        _check(L().fdk_nn_head_bwd(_p(dout), _p(k["d1"]), _p(self.hw), _p(dd1), _p(self.ghw),
-                                   _p(self.ghb), dout.numel(), c1, _st()), "head_bwd")
+        if self.shuffle:
+            head_shuffle_bwd(dout, k["d1"], self.hw, dd1, self.ghw, self.ghb)
+        else:
+            _check(L().fdk_nn_head_bwd(_p(dout), _p(k["d1"]), _p(self.hw), _p(dd1), _p(self.ghw),
+                                       _p(self.ghb), dout.numel(), c1, _st()), "head_bwd")
         dd1a = self.d1b.backward(dd1, k["d1a"], k["d1"], s1_)
         dcat1 = self.d1a.backward(dd1a, k["cat1"], k["d1a"], s1_)
         dd2 = upsample(dcat1, torch.empty(s2_ + (c2,), dtype=I16, device=dev), s2_, c2, c2 + c1,
@@ -330,9 +420,21 @@ class UNet3D:
         self.cache = None
         return loss  # sum of squared errors (device)
 
+    def check_shape(self, shape):
+        """Raise unless an (N, D, H, W) input fits the network's kernels."""
+        _, D, H, W = shape
+        need = (8, 32, 256) if self.shuffle else (4, 16, 128)
+        if D % need[0] or H % need[1] or W % need[2]:
+            raise ValueError(f"U-Net input (D, H, W) = {(D, H, W)} must be multiples of {need}")
+
+    @property
+    def min_shape(self):
+        """The smallest (D, H, W) input."""
+        return (8, 32, 256) if self.shuffle else (4, 16, 128)
+
     # ------------------------------------------------------------ persistence
     def state_dict(self):
-        d = {}
+        d = {"arch.shuffle": np.array(int(self.shuffle), np.int32)}
         for i, cv in enumerate(self.convs):
             d[f"conv{i}.w"] = cv.w.cpu().numpy()
             d[f"conv{i}.b"] = cv.b.cpu().numpy()
@@ -342,6 +444,10 @@ class UNet3D:
 
     def load_state_dict(self, d):
         torch = _torch()
+        shuffle = bool(int(d["arch.shuffle"])) if "arch.shuffle" in d else False
+        if shuffle != self.shuffle:
+            raise ValueError(f"weights are for shuffle={shuffle}, the network has "
+                             f"shuffle={self.shuffle}")
         for i, cv in enumerate(self.convs):
             cv.w.copy_(torch.as_tensor(d[f"conv{i}.w"]))
             cv.b.copy_(torch.as_tensor(d[f"conv{i}.b"]))
@@ -355,3 +461,14 @@ class UNet3D:
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
             self.load_state_dict({k: z[k] for k in z.files})
+
+    @classmethod
+    def from_file(cls, path, device="cuda"):
+        """A network built for the saved weights' architecture, with them loaded."""
+        with np.load(path, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        shuffle = bool(int(d["arch.shuffle"])) if "arch.shuffle" in d else False
+        base = int(d["conv1.w"].shape[1])
+        net = cls(base=base, seed=0, device=device, shuffle=shuffle)
+        net.load_state_dict(d)
+        return net

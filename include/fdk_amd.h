@@ -288,6 +288,18 @@ int fdk_nn_head(const void* h, const float* w, const float* b, const float* in, 
                 int64_t nvox, int c, void* stream);
 int fdk_nn_head_bwd(const float* dout, const void* h, const float* w, void* dh, float* dw,
                     float* db, int64_t nvox, int c, void* stream);
+/* Space-to-depth input: FP32 x[n][d][h][w] -> BF16 y[n][d/2][h/2][w/2][cy], the
+ * 2x2x2 block (2z+iz, 2y+iy, 2x+ix) as channels 4 iz + 2 iy + ix, channels
+ * 8..cy-1 zero.  d, h, w even; cy % 8 == 0. */
+int fdk_nn_unshuffle2(const float* x, void* y, int n, int d, int h, int w, int cy, void* stream);
+/* Depth-to-space head over the same block map: for half-resolution voxel v and
+ * k = 4 iz + 2 iy + ix, out[block k] = in[block k] + b[k] + sum_c w[k][c] h[v][c]
+ * (w [8][c], b [8], FP32; d, h, w the full-resolution sizes, even; c == 32),
+ * and its adjoint (dh BF16; dw [8][c], db [8] accumulated). */
+int fdk_nn_head_shuffle(const void* h, const float* w, const float* b, const float* in, float* out,
+                        int n, int d, int h_, int w_, int c, void* stream);
+int fdk_nn_head_shuffle_bwd(const float* dout, const void* h, const float* w, void* dh, float* dw,
+                            float* db, int n, int d, int h_, int w_, int c, void* stream);
 /* loss_sum += sum (out - t)^2; dout = 2 (out - t) / nvox when dout != NULL. */
 int fdk_nn_mse(const float* out, const float* target, float* dout, float* loss_sum, int64_t nvox,
                void* stream);

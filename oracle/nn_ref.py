@@ -83,3 +83,32 @@ def upsample2(x):
 def upsample2_adjoint(dy):
     n, D, H, W, c = dy.shape
     return dy.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2, c).sum(axis=(2, 4, 6))
+
+
+def unshuffle2(x, cy=8):
+    """Space-to-depth: FP32 [n][2d][2h][2w] -> [n][d][h][w][cy]; channel
+    4 iz + 2 iy + ix holds voxel (2z + iz, 2y + iy, 2x + ix), channels 8.. zero."""
+    n, D, H, W = x.shape
+    b = x.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2).transpose(0, 1, 3, 5, 2, 4, 6)
+    out = np.zeros((n, D // 2, H // 2, W // 2, cy))
+    out[..., :8] = b.reshape(n, D // 2, H // 2, W // 2, 8)
+    return out
+
+
+def shuffle2(y):
+    """Depth-to-space, the inverse of unshuffle2 on channels 0..7."""
+    n, d, h, w = y.shape[:4]
+    b = y[..., :8].reshape(n, d, h, w, 2, 2, 2).transpose(0, 1, 4, 2, 5, 3, 6)
+    return b.reshape(n, 2 * d, 2 * h, 2 * w)
+
+
+def head_shuffle(h, w, b, x):
+    """out = x + shuffle2(h w^T + b): h [n][d][h][w][c], w [8][c], b [8]."""
+    return x + shuffle2(h @ w.T + b)
+
+
+def head_shuffle_adjoint(dout, h, w):
+    """(dh, dw, db) of head_shuffle for the output gradient dout."""
+    g = unshuffle2(dout)
+    c = h.shape[-1]
+    return g @ w, g.reshape(-1, 8).T @ h.reshape(-1, c), g.reshape(-1, 8).sum(0)

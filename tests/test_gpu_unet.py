@@ -160,6 +160,38 @@ def test_pool_upsample_copy_head_mse_adam(torch_cuda):
     assert np.abs(from_bf(wbf) - R.bf16_round(tw.cpu().numpy())).max() == 0.0
 
 
+@pytest.mark.parametrize("full", [(1, 6, 10, 14), (1, 128, 128, 136)])
+def test_space_to_depth_kernels(torch_cuda, full):
+    """unshuffle2 bit-exact; the 8-output head and its adjoint against the FP64
+    oracle (the second shape runs the adjoint's tile loop more than once)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(12)
+    n, D, H, W = full
+    half = (n, D // 2, H // 2, W // 2)
+    x = rng.standard_normal(full).astype(np.float32)
+    xd = torch.as_tensor(x).cuda()
+    y = torch.empty(half + (32,), dtype=torch.int16, device="cuda")
+    U.unshuffle2(xd, y)
+    assert np.array_equal(from_bf(y), R.unshuffle2(R.bf16_round(x), 32))
+    h = R.bf16_round(rng.standard_normal(half + (32,)))
+    w = rng.standard_normal((8, 32)).astype(np.float32)
+    b = rng.standard_normal(8).astype(np.float32)
+    hd, wd, bd = bf(torch, h), torch.as_tensor(w).cuda(), torch.as_tensor(b).cuda()
+    out = torch.empty(full, device="cuda")
+    U.head_shuffle(hd, wd, bd, xd, out)
+    assert rel(out.cpu().numpy(), R.head_shuffle(h, w.astype(np.float64), b, x)) < 1e-6
+    dout = rng.standard_normal(full).astype(np.float32)
+    dh = torch.empty_like(hd)
+    dw = torch.zeros((8, 32), device="cuda")
+    db = torch.zeros(8, device="cuda")
+    U.head_shuffle_bwd(torch.as_tensor(dout).cuda(), hd, wd, dh, dw, db)
+    torch.cuda.synchronize()
+    rdh, rdw, rdb = R.head_shuffle_adjoint(dout.astype(np.float64), h, w.astype(np.float64))
+    assert rel(from_bf(dh), rdh) < 4e-3  # BF16 output
+    assert rel(dw.cpu().numpy(), rdw) < 1e-5
+    assert rel(db.cpu().numpy(), rdb) < 1e-5
+
+
 def _torch_unet(net, x):
     """The same network in torch float64 on the CPU, weights taken from net."""
     import torch
@@ -181,7 +213,11 @@ def _torch_unet(net, x):
         params[id(cv)] = (w, b)
         return torch.relu(Fn.conv3d(h, w, b, padding=1))
 
-    xt = torch.as_tensor(R.bf16_round(x)[:, None]).requires_grad_(False)
+    if net.shuffle:
+        xt = torch.as_tensor(np.ascontiguousarray(
+            R.unshuffle2(R.bf16_round(x), 32).transpose(0, 4, 1, 2, 3)))
+    else:
+        xt = torch.as_tensor(R.bf16_round(x)[:, None]).requires_grad_(False)
     e1 = conv(net.e1b, conv(net.e1a, xt))
     e2 = conv(net.e2b, conv(net.e2a, Fn.avg_pool3d(e1, 2)))
     bt = conv(net.bb, conv(net.ba, Fn.avg_pool3d(e2, 2)))
@@ -189,19 +225,27 @@ def _torch_unet(net, x):
     d1 = conv(net.d1b, conv(net.d1a, torch.cat([Fn.interpolate(d2, scale_factor=2), e1], 1)))
     hw = torch.as_tensor(net.hw.cpu().numpy().astype(np.float64)).requires_grad_(True)
     hb = torch.as_tensor(net.hb.cpu().numpy().astype(np.float64)).requires_grad_(True)
+    if net.shuffle:  # 8 outputs per half-resolution voxel, block k = 4 iz + 2 iy + ix
+        s = torch.einsum("ncdhw,kc->nkdhw", d1, hw) + hb[None, :, None, None, None]
+        n, _, d, h, w = s.shape
+        s = s.reshape(n, 2, 2, 2, d, h, w).permute(0, 4, 1, 5, 2, 6, 3)
+        out = torch.as_tensor(x.astype(np.float64)) + s.reshape(n, 2 * d, 2 * h, 2 * w)
+        return out, params, hw, hb
     out = torch.as_tensor(x[:, None].astype(np.float64)) + \
         torch.einsum("ncdhw,c->ndhw", d1, hw)[:, None] + hb
     return out[:, 0], params, hw, hb
 
 
-def test_unet_forward_and_gradients_match_torch(torch_cuda):
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_unet_forward_and_gradients_match_torch(torch_cuda, shuffle):
     torch = torch_cuda
-    net = U.UNet3D(base=32, seed=3)
+    net = U.UNet3D(base=32, seed=3, shuffle=shuffle)
     with torch.no_grad():
         net.hw.mul_(300.0)  # a head large enough that the net's output is not just x
     rng = np.random.default_rng(4)
-    x = rng.standard_normal((1, 8, 16, 128)).astype(np.float32)
-    tgt = rng.standard_normal((1, 8, 16, 128)).astype(np.float32)
+    shape = (1, 8, 32, 256) if shuffle else (1, 8, 16, 128)
+    x = rng.standard_normal(shape).astype(np.float32)
+    tgt = rng.standard_normal(shape).astype(np.float32)
     xd = torch.as_tensor(x).cuda()
     out = net.forward(xd, keep=True)
     dout = torch.empty_like(out)
@@ -233,17 +277,20 @@ def test_unet_forward_and_gradients_match_torch(torch_cuda):
     print("weight-gradient cosine similarity per conv:", [round(c, 5) for c in cos])
 
 
-def test_unet_training_reduces_loss(torch_cuda):
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_unet_training_reduces_loss(torch_cuda, shuffle):
     torch = torch_cuda
-    net = U.UNet3D(base=32, seed=5)
+    net = U.UNet3D(base=32, seed=5, shuffle=shuffle)
     rng = np.random.default_rng(6)
     # a smooth target and a noisy, blurred input: the net must learn a filter
-    z, y, x = np.meshgrid(np.linspace(-1, 1, 16), np.linspace(-1, 1, 16), np.linspace(-1, 1, 128),
+    d, h, w = (16, 32, 256) if shuffle else (16, 16, 128)
+    z, y, x = np.meshgrid(np.linspace(-1, 1, d), np.linspace(-1, 1, h), np.linspace(-1, 1, w),
                           indexing="ij")
     clean = (np.sin(3 * x) * np.cos(2 * y) + z)[None].astype(np.float32)
     clean = np.concatenate([clean, -clean])
     noisy = clean + 0.3 * rng.standard_normal(clean.shape).astype(np.float32)
     xd, td = torch.as_tensor(noisy).cuda(), torch.as_tensor(clean).cuda()
-    losses = [net.train_step(xd, td, lr=1e-3).item() for _ in range(40)]
-    print("training loss:", [round(l, 1) for l in losses[::8]], "->", round(losses[-1], 1))
+    steps = 120 if shuffle else 40  # the half-resolution net starts further from a smoother
+    losses = [net.train_step(xd, td, lr=1e-3).item() for _ in range(steps)]
+    print("training loss:", [round(l, 1) for l in losses[::steps // 5]], "->", round(losses[-1], 1))
     assert losses[-1] < 0.5 * losses[0]

@@ -74,3 +74,41 @@ def test_bf16_rounding():
     assert list(r[:4]) == [1.0, 1.0, 1.0078125, 1.015625]
     assert np.allclose(R.from_bf16_bits(R.to_bf16_bits(vals)), r)
     assert np.abs(r - vals).max() <= 2 ** -8 * np.abs(vals).max()
+
+
+def test_space_to_depth_index_map_and_round_trip():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 4, 6, 8))
+    y = R.unshuffle2(x, 16)
+    assert y.shape == (2, 2, 3, 4, 16) and not y[..., 8:].any()
+    for n, z, yy, xx, iz, iy, ix in np.ndindex(2, 2, 3, 4, 2, 2, 2):
+        assert y[n, z, yy, xx, 4 * iz + 2 * iy + ix] == x[n, 2 * z + iz, 2 * yy + iy, 2 * xx + ix]
+    assert np.array_equal(R.shuffle2(y), x)
+    # a head whose weights pick channel j of h puts h[..., j] on block j
+    h = rng.standard_normal((1, 2, 3, 4, 32))
+    w = np.zeros((8, 32))
+    w[np.arange(8), np.arange(8) + 3] = 1.0
+    out = R.head_shuffle(h, w, np.zeros(8), np.zeros((1, 4, 6, 8)))
+    assert np.array_equal(out, R.shuffle2(h[..., 3:11]))
+
+
+def test_space_to_depth_head_adjoint_matches_autograd():
+    rng = np.random.default_rng(6)
+    h = rng.standard_normal((2, 2, 3, 4, 32))
+    w = rng.standard_normal((8, 32))
+    b = rng.standard_normal(8)
+    x = rng.standard_normal((2, 4, 6, 8))
+    dout = rng.standard_normal((2, 4, 6, 8))
+    th, tw, tb = (torch.as_tensor(a).requires_grad_(True) for a in (h, w, b))
+    # depth-to-space through torch's own 2-D pixel_shuffle, applied per plane pair
+    s = (th @ tw.T + tb)                                     # n d h w 8
+    s = s.permute(0, 1, 4, 2, 3)                             # n d 8 h w, channel 4 iz + (2 iy + ix)
+    s = s.reshape(2, 2, 2, 4, 3, 4)                          # n d iz (iy ix) h w
+    planes = F.pixel_shuffle(s.reshape(2 * 2 * 2, 4, 3, 4), 2)  # (n d iz) 1 2h 2w
+    out = torch.as_tensor(x) + planes.reshape(2, 2, 2, 6, 8).reshape(2, 4, 6, 8)
+    assert np.abs(out.detach().numpy() - R.head_shuffle(h, w, b, x)).max() < 1e-12
+    (out * torch.as_tensor(dout)).sum().backward()
+    dh, dw, db = R.head_shuffle_adjoint(dout, h, w)
+    assert np.abs(dh - th.grad.numpy()).max() < 1e-12
+    assert np.abs(dw - tw.grad.numpy()).max() < 1e-10
+    assert np.abs(db - tb.grad.numpy()).max() < 1e-12
