@@ -311,6 +311,236 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
       }
 }
 
+// Lean form of the LDS-halo kernel, used for cout = 32 (NT = 2; at NT = 4 its
+// registers spill), on the same
+// tiles (2 planes x 4 rows x 32 voxels per block, 1 plane x 2 rows per wave)
+// and halo layout as conv3d_fwd3_kernel:
+//  * the 27 taps are unrolled, so every A-fragment read is one ds_read_b128
+//    at an immediate offset from a per-lane base;
+//  * weights come through a buffer resource (lane offset in a VGPR, tap and
+//    chunk offset in an SGPR), P taps ahead of their MFMAs;
+//  * the MFMA takes the weights as its A operand and the voxels as B, so lane
+//    (m, kq) ends with output channels 4 kq .. 4 kq + 3 of voxel m: one 8-byte
+//    BF16 store (16-byte FP32) per lane and tile instead of four 2-byte ones;
+//  * a block walks `tpb` consecutive tiles; the next (tile, chunk) halo is
+//    loaded into registers while the current one's taps run.
+// Staging offsets are 32-bit (4 planes of the input fit in 2^31 elements).
+constexpr int kHaloPer = (kHaloChunks + 255) / 256;  // halo slots per thread
+constexpr int kWeightRsrcWord3 = 0x00020000;  // raw (untyped) buffer access, gfx9 format word
+constexpr int kHaloRowSlots = 4 * kHaloX;            // 16-byte slots per halo row
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline uint2 pack4_bf16(float a, float b, float c, float d) {
+  bf16x2 lo, hi;
+  lo[0] = (__bf16)a;  // v_cvt_pk_bf16_f32: round to nearest even
+  lo[1] = (__bf16)b;
+  hi[0] = (__bf16)c;
+  hi[1] = (__bf16)d;
+  return make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
+}
+
+struct Fwd4Tile {
+  int x0, y0, z0, n, zb;  // zb: n * d + z0
+  bool interior;          // the whole halo lies inside the volume
+};
+
+__device__ __forceinline__ Fwd4Tile fwd4_tile(const Conv& a, int tile, int tiles_x, int tiles_y) {
+  Fwd4Tile f;
+  const int tx = tile % tiles_x, r = tile / tiles_x;
+  const int ty = r % tiles_y;
+  f.zb = (r / tiles_y) * 2;
+  f.x0 = tx * 32;
+  f.y0 = ty * 4;
+  f.n = f.zb / a.d;
+  f.z0 = f.zb - f.n * a.d;
+  f.interior = f.z0 >= 1 && f.z0 + 3 <= a.d && f.y0 >= 1 && f.y0 + 5 <= a.h && f.x0 >= 1 &&
+               f.x0 + 33 <= a.w;
+  return f;
+}
+
+// Loads the halo of (tile, 32-channel chunk c0) into registers: slot s = t + 256 j
+// is halo row s / 136 (plane-major), x s % 136 / 4 and 8-channel group s % 4.
+// Interior tiles (uniform) skip the range checks.
+__device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a, const Fwd4Tile& f,
+                                           int c0) {
+  const int t = threadIdx.x;
+  const u16* org = a.x + ((((long long)f.n * a.d + f.z0 - 1) * a.h + f.y0 - 1) * a.w + f.x0 - 1) *
+                             a.cin + c0;
+  const unsigned plane = (unsigned)a.h * (unsigned)a.w * (unsigned)a.cin;
+  const unsigned rowe = (unsigned)a.w * (unsigned)a.cin;
+  int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
+  if (f.interior) {
+#pragma unroll
+    for (int j = 0; j < kHaloPer; ++j) {
+      const int hz = row / kHaloRows, hy = row - hz * kHaloRows;
+      const unsigned off = hz * plane + hy * rowe + (unsigned)(w >> 2) * a.cin + 8 * (w & 3);
+      if (j + 1 < kHaloPer || t + 256 * j < kHaloChunks)
+        pre[j] = *reinterpret_cast<const uint4*>(org + off);
+      w += 256 - kHaloRowSlots;  // 256 = one row (136 slots) + 120
+      row += 1;
+      if (w >= kHaloRowSlots) {
+        w -= kHaloRowSlots;
+        row += 1;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < kHaloPer; ++j) {
+    const int hz = row / kHaloRows, hy = row - hz * kHaloRows, hx = w >> 2;
+    const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
+    const bool ok = (j + 1 < kHaloPer || t + 256 * j < kHaloChunks) && zz >= 0 && zz < a.d &&
+                    yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    const unsigned off = hz * plane + hy * rowe + (unsigned)hx * a.cin + 8 * (w & 3);
+    pre[j] = ok ? *reinterpret_cast<const uint4*>(org + off) : make_uint4(0u, 0u, 0u, 0u);
+    w += 256 - kHaloRowSlots;
+    row += 1;
+    if (w >= kHaloRowSlots) {
+      w -= kHaloRowSlots;
+      row += 1;
+    }
+  }
+}
+
+__device__ __forceinline__ void fwd4_commit(uint4* halo, const uint4 (&pre)[kHaloPer]) {
+  const int t = threadIdx.x;
+  int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
+#pragma unroll
+  for (int j = 0; j < kHaloPer; ++j) {
+    if (t + 256 * j < kHaloChunks) halo[(row * 4 + (w & 3)) * kHaloX + (w >> 2)] = pre[j];
+    w += 256 - kHaloRowSlots;
+    row += 1;
+    if (w >= kHaloRowSlots) {
+      w -= kHaloRowSlots;
+      row += 1;
+    }
+  }
+}
+
+// Weight fragments of one tap at the running offset `soff` (advanced by one tap;
+// the empty asm keeps it a running SGPR instead of 27 hoisted products).
+template <int NT>
+__device__ __forceinline__ void fwd4_load_w(bf16x8 (&b)[NT], __amdgpu_buffer_rsrc_t wr,
+                                            unsigned lane_off, unsigned& soff, unsigned nt_bytes,
+                                            unsigned tap_bytes) {
+  unsigned o = soff;
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    b[nt] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(wr, lane_off, o, 0));
+    o += nt_bytes;
+    asm volatile("" : "+s"(o));
+  }
+  soff += tap_bytes;
+  asm volatile("" : "+s"(soff));
+}
+
+// The 4 A fragments (voxels) of one tap from the halo, at immediate offsets.
+__device__ __forceinline__ void fwd4_read_a(bf16x8 (&af)[4], const uint4* hb, int tap) {
+  constexpr int kRowStride = 4 * kHaloX;
+  const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // 0..2: offset + 1
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+    af[mt] = __builtin_bit_cast(
+        bf16x8, hb[(dz * kHaloRows + (mt >> 1) + dy) * kRowStride + 16 * (mt & 1) + dx]);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_fwd4_kernel(
+    Conv a, int tiles_x, int tiles_y, int ntiles, int tpb) {
+  constexpr int P = NT == 2 ? 3 : 2;  // weight prefetch distance in taps
+  constexpr int R = P + 1;            // weight ring slots
+  __shared__ uint4 halo[kHaloChunks];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int pz = wave >> 1, py = 2 * (wave & 1);  // the wave's plane and first row in the tile
+  const uint4* hb = halo + ((pz * kHaloRows + py) * 4 + kq) * kHaloX + m;
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), kWeightRsrcWord3);
+  const unsigned lane_off = (unsigned)(kq * 8 + m * a.cin) * 2u;
+  const unsigned tap_bytes = (unsigned)(a.cout * a.cin) * 2u, nt_bytes = (unsigned)(16 * a.cin) * 2u;
+  const int nchunks = a.cin / 32;
+  const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
+  uint4 pre[kHaloPer];
+  if (first < last) fwd4_fetch(pre, a, fwd4_tile(a, first, tiles_x, tiles_y), 0);
+  for (int tile = first; tile < last; ++tile) {
+    const Fwd4Tile f = fwd4_tile(a, tile, tiles_x, tiles_y);
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ch = 0; ch < nchunks; ++ch) {
+      __syncthreads();  // every wave is done reading the previous halo
+      fwd4_commit(halo, pre);
+      __syncthreads();
+      unsigned soff = (unsigned)ch * 64u;  // 32 channels x 2 bytes; then + tap x tap_bytes
+      bf16x8 b[R][NT];
+#pragma unroll
+      for (int tp = 0; tp < P; ++tp) fwd4_load_w<NT>(b[tp], wr, lane_off, soff, nt_bytes, tap_bytes);
+      // the next halo's loads run under this chunk's taps
+      {
+        int nt_ = tile, nc = ch + 1;
+        if (nc == nchunks) {
+          nc = 0;
+          ++nt_;
+        }
+        if (nt_ < last) fwd4_fetch(pre, a, nt_ == tile ? f : fwd4_tile(a, nt_, tiles_x, tiles_y), nc * 32);
+      }
+      // NT == 2: A fragments are read one tap ahead of their MFMAs (two register
+      // sets); NT == 4 has no registers to spare and reads them in the tap.
+      constexpr int AB = NT == 2 ? 2 : 1;
+      bf16x8 af[AB][4];
+      if (AB == 2) fwd4_read_a(af[0], hb, 0);
+#pragma unroll
+      for (int tap = 0; tap < 27; ++tap) {
+        if (tap + P < 27)
+          fwd4_load_w<NT>(b[(tap + P) % R], wr, lane_off, soff, nt_bytes, tap_bytes);
+        if (AB == 2 && tap + 1 < 27) fwd4_read_a(af[(tap + 1) % AB], hb, tap + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (AB == 1) fwd4_read_a(af[0], hb, tap);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt)
+            acc[mt][nt] = mfma(b[tap % R][nt], af[tap % AB][mt], acc[mt][nt]);
+      }
+    }
+    // lane (m, kq) holds channels nt * 16 + 4 kq + i of voxel m of M tile mt
+    const int zo = f.zb + pz;
+    float4 bias[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + nt * 16 + 4 * kq)
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const long long vox =
+          ((long long)zo * a.h + f.y0 + py + (mt >> 1)) * a.w + f.x0 + 16 * (mt & 1) + m;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const long long o = vox * a.cout + nt * 16 + 4 * kq;
+        float v[4] = {acc[mt][nt][0] + bias[nt].x, acc[mt][nt][1] + bias[nt].y,
+                      acc[mt][nt][2] + bias[nt].z, acc[mt][nt][3] + bias[nt].w};
+        if (a.relu)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = fmaxf(v[i], 0.0f);
+        if (a.resid) {
+          const float4 r = *reinterpret_cast<const float4*>(a.resid + o);
+          v[0] += r.x;
+          v[1] += r.y;
+          v[2] += r.z;
+          v[3] += r.w;
+        }
+        if (a.out_f32)
+          *reinterpret_cast<float4*>(static_cast<float*>(a.y) + o) = make_float4(v[0], v[1], v[2], v[3]);
+        else
+          *reinterpret_cast<uint2*>(static_cast<u16*>(a.y) + o) = pack4_bf16(v[0], v[1], v[2], v[3]);
+      }
+    }
+  }
+}
+
 // cin == 1 (the network input): the 27 taps of the single channel form K (one
 // MFMA K step, padded to 32); weights [cout][32].
 template <int NT>
@@ -943,18 +1173,21 @@ extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void*
   return FDK_OK;
 }
 
-// Forward convolution kernel: the LDS-staged halo kernel where its shape
-// rules hold (default), else the pipelined 2-row tile, else the 1-row kernel;
-// FDK_NN_CONV=2 or 1 caps the choice.
+// Forward convolution kernel: the lean LDS-halo kernel (cout 32) or the
+// LDS-halo kernel where their shape rules hold (default), else the pipelined
+// 2-row tile, else the 1-row kernel; FDK_NN_CONV=3, 2 or 1 caps the choice.
 static int conv_kernel_choice() {
   static const int k = [] {
     const char* e = std::getenv("FDK_NN_CONV");
     if (e && e[0] == '1') return 1;
     if (e && e[0] == '2') return 2;
-    return 3;
+    if (e && e[0] == '3') return 3;
+    return 4;
   }();
   return k;
 }
+
+static bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
 extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid,
                                  void* y, int n, int d, int h, int wd, int cin, int cout, int relu,
@@ -970,7 +1203,19 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   dim3 grid(wd / 32, h / 4, n * d);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ck = conv_kernel_choice();
-  const bool staged = cin % 32 == 0 && d % 2 == 0 && cout <= 128 && ck == 3;
+  const bool lean = ck >= 4 && cin % 32 == 0 && d % 2 == 0 && cout == 32 &&
+                    aligned16(x) && aligned16(w) && aligned16(y) && (!bias || aligned16(bias)) &&
+                    (!resid || aligned16(resid)) &&
+                    ((4LL * h * wd + 64) * cin < (1LL << 31)) && 27LL * cout * cin * 2 < (1LL << 31);
+  if (lean) {
+    const int tiles_x = wd / 32, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
+    const int tpb = 2;
+    const dim3 g4((ntiles + tpb - 1) / tpb);
+    hipLaunchKernelGGL(conv3d_fwd4_kernel<2>, g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
+  const bool staged = cin % 32 == 0 && d % 2 == 0 && cout <= 128 && ck >= 3;
   const bool tiled = !staged && cin % 32 == 0 && h % 8 == 0 && cout <= 128 && ck >= 2;
   dim3 grid2(wd / 32, h / 8, n * d);
   dim3 grid3(wd / 32, h / 4, n * d / 2);

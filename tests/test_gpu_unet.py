@@ -62,6 +62,30 @@ def test_conv3d_forward(torch_cuda, cin, cout, W):
     assert rel(y32.cpu().numpy(), R.conv3d(x, w) + res) < 1e-5
 
 
+@pytest.mark.parametrize("cin,shape", [(32, (2, 6, 12, 96)), (96, (2, 6, 12, 96)),
+                                       (64, (1, 2, 4, 96))])
+def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
+    """Even-depth shapes, which take the LDS-halo kernels (cout 32: the lean one,
+    two tiles per block): interior and border tiles, both batch items, one to
+    three 32-channel chunks, and an odd tile count (3 tiles: the last block has
+    one)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(cin + shape[0])
+    for cout in (32, 64):
+        x = R.bf16_round(rng.standard_normal(shape + (cin,)))
+        w = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+        b = rng.standard_normal(cout).astype(np.float32)
+        y = torch.empty(shape + (cout,), dtype=torch.int16, device="cuda")
+        U.conv3d(bf(torch, x), bf(torch, w), torch.as_tensor(b).cuda(), y, shape, cin, cout,
+                 relu=True)
+        assert rel(from_bf(y), np.maximum(R.conv3d(x, w, b), 0)) < 4e-3
+        res = rng.standard_normal(shape + (cout,)).astype(np.float32)
+        y32 = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
+        U.conv3d(bf(torch, x), bf(torch, w), None, y32, shape, cin, cout, relu=False,
+                 out_f32=True, resid=torch.as_tensor(res).cuda())
+        assert rel(y32.cpu().numpy(), R.conv3d(x, w) + res) < 1e-5
+
+
 def test_conv3d_forward_single_input_channel(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(5)
