@@ -182,23 +182,42 @@ def fp_stage_256cube():
             "bytes_per_sample": 32}
 
 
+LEARNED_WEIGHTS = [  # (name, file): the speed-oriented network first
+    ("s2d4", "unet_sparse1024_s2d4.npz"),
+    ("s2d2", "unet_sparse1024_s2d2.npz"),
+    ("full_res", "unet_sparse1024.npz"),
+]
+
+
 def learned_sparse_view():
     """The learned sparse-view comparison on a held-out phantom: FDK from all
     1024 noisy views of a 1024^3 scan against FDK from every 4th view followed
-    by the trained U-Net (fdk_amd/weights/unet_sparse1024.npz), with GPU wall times
-    (projections already in HBM) and RMSE / SSIM against the 1024-view FDK and
-    against the phantom."""
+    by each trained U-Net in fdk_amd/weights (space-to-depth edge 4, edge 2, and
+    the full-resolution network), with GPU wall times (projections already in
+    HBM) and RMSE / SSIM against the 1024-view FDK and against the phantom.
+    The first network present is the headline (speedup, RMSE, SSIM at top
+    level)."""
     from fdk_amd import learned as Lr
     from fdk_amd.unet import UNet3D
-    w = os.path.join(HERE, "fdk_amd", "weights", "unet_sparse1024.npz")
-    if not os.path.exists(w):
+    nets = {}
+    for name, fname in LEARNED_WEIGHTS:
+        w = os.path.join(HERE, "fdk_amd", "weights", fname)
+        if os.path.exists(w):
+            nets[name] = (UNet3D.from_file(w), "fdk_amd/weights/" + fname)
+    if not nets:
         return None
     p = Lr.geometry(1024, 1024)
     sc = Lr.Scanner(p)
-    net = UNet3D(base=32, seed=0)
-    net.load(w)
-    r = Lr.compare(net, sc, 100000)
-    r["weights"] = "fdk_amd/weights/unet_sparse1024.npz"
+    r = Lr.compare_many({k: v[0] for k, v in nets.items()}, sc, 100000)
+    for k, v in nets.items():
+        r["networks"][k]["weights"] = v[1]
+    head = r["networks"][next(iter(nets))]
+    r["headline_network"] = next(iter(nets))
+    for key in ("t_network_s", "t_learned_s", "speedup_vs_full_fdk"):
+        r[key] = head[key]
+    r["rmse_vs_full_fdk"]["sparse_fdk+net"] = head["rmse_vs_full_fdk"]
+    r["rmse_vs_phantom"]["sparse_fdk+net"] = head["rmse_vs_phantom"]
+    r["ssim_vs_full_fdk"]["sparse_fdk+net"] = head["ssim_vs_full_fdk"]
     r["phantom_seed"] = 100000
     return r
 

@@ -34,6 +34,13 @@ __device__ inline u16 f2bf(float f) {  // round to nearest even
   return (u16)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
 }
 
+// Activation of a convolution's output: 0 none, 1 ReLU, 2 leaky ReLU (slope
+// kLeak for negative inputs).
+constexpr float kLeak = 0.01f;
+__device__ inline float activate(float v, int mode) {
+  return mode == 1 ? fmaxf(v, 0.0f) : mode == 2 ? (v > 0.0f ? v : kLeak * v) : v;
+}
+
 __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -71,7 +78,7 @@ __device__ inline void conv_epilogue(const Conv& a, f32x4 (&acc)[2][NT], int nzy
         const long long o = ((long long)nzy * a.w + x) * a.cout + co;
         float v = acc[mt][nt][i];
         if (a.bias) v += a.bias[co];
-        if (a.relu) v = fmaxf(v, 0.0f);
+        v = activate(v, a.relu);
         if (a.resid) v += a.resid[o];
         if (a.out_f32) static_cast<float*>(a.y)[o] = v;
         else static_cast<u16*>(a.y)[o] = f2bf(v);
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(256) void conv3d_fwd2_kernel(Conv a) {
         const long long o = (((long long)nz * a.h + y0 + (mt >> 1)) * a.w + x) * a.cout + co;
         float v = acc[mt][nt][i];
         if (a.bias) v += a.bias[co];
-        if (a.relu) v = fmaxf(v, 0.0f);
+        v = activate(v, a.relu);
         if (a.resid) v += a.resid[o];
         if (a.out_f32) static_cast<float*>(a.y)[o] = v;
         else static_cast<u16*>(a.y)[o] = f2bf(v);
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
             (((long long)zo * a.h + y0 + py + (mt >> 1)) * a.w + x) * a.cout + co;
         float v = acc[mt][nt][i];
         if (a.bias) v += a.bias[co];
-        if (a.relu) v = fmaxf(v, 0.0f);
+        v = activate(v, a.relu);
         if (a.resid) v += a.resid[o];
         if (a.out_f32) static_cast<float*>(a.y)[o] = v;
         else static_cast<u16*>(a.y)[o] = f2bf(v);
@@ -522,9 +529,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         const long long o = vox * a.cout + nt * 16 + 4 * kq;
         float v[4] = {acc[mt][nt][0] + bias[nt].x, acc[mt][nt][1] + bias[nt].y,
                       acc[mt][nt][2] + bias[nt].z, acc[mt][nt][3] + bias[nt].w};
-        if (a.relu)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = fmaxf(v[i], 0.0f);
+        for (int i = 0; i < 4; ++i) v[i] = activate(v[i], a.relu);
         if (a.resid) {
           const float4 r = *reinterpret_cast<const float4*>(a.resid + o);
           v[0] += r.x;
@@ -713,9 +719,11 @@ __global__ void bias_grad_kernel(const u16* dy, float* db, long long nvox, int c
   atomicAdd(&db[ch], s);
 }
 
-__global__ void relu_bwd_kernel(const u16* dy, const u16* y, u16* dz, long long n) {
+// Adjoint of the activation at output y (mode 1 ReLU, 2 leaky ReLU: y > 0
+// exactly where the input was).
+__global__ void relu_bwd_kernel(const u16* dy, const u16* y, u16* dz, long long n, int mode) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    dz[i] = (bf2f(y[i]) > 0.f) ? dy[i] : (u16)0;
+    dz[i] = (bf2f(y[i]) > 0.f) ? dy[i] : mode == 2 ? f2bf(kLeak * bf2f(dy[i])) : (u16)0;
 }
 
 // 2x2x2 average pooling (and its adjoint) on [n][d][h][w][c].
@@ -909,50 +917,80 @@ __global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w,
   if (threadIdx.x == 0) atomicAdd(db, red[c]);
 }
 
-// Space-to-depth at the network's ends.  An FP32 volume [n][2d][2h][2w] maps to
-// the half-resolution grid [n][d][h][w][cy] BF16: voxel (z, y, x) holds the
-// 2x2x2 block (2z + iz, 2y + iy, 2x + ix) as channels 4 iz + 2 iy + ix, and
-// channels 8..cy-1 are zero.  One thread per half-resolution voxel; the two
-// x-neighbours of a block row are one 8-byte load.
+// Space-to-depth at the network's ends, block edge F (2 or 4).  An FP32 volume
+// [n][F d][F h][F w] maps to the coarse grid [n][d][h][w][cy] BF16: voxel
+// (z, y, x) holds the F^3 block (F z + iz, F y + iy, F x + ix) as channels
+// (iz F + iy) F + ix, and channels F^3..cy-1 are zero.  One thread per coarse
+// voxel; each block row of F samples is one F x 4-byte load.
 __device__ inline unsigned pack_bf2(float lo, float hi) {
   return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
 }
 
-__global__ __launch_bounds__(256) void unshuffle2_kernel(const float* x, uint4* y, int h2, int w2,
-                                                         int cyq, long long nvox) {
-  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nvox) return;
-  const int xx = (int)(v % w2);
-  const long long r = v / w2;  // (n d + z) h + y of the half-resolution voxel
-  const int yy = (int)(r % h2);
-  const long long nz = r / h2;
-  const long long W = 2LL * w2, H = 2LL * h2;
-  const float* base = x + ((2 * nz) * H + 2 * yy) * W + 2 * xx;
-  float2 q[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)  // k = 2 iz + iy
-    q[k] = *reinterpret_cast<const float2*>(base + ((k >> 1) * H + (k & 1)) * W);
-  uint4* o = y + v * cyq;
-  o[0] = make_uint4(pack_bf2(q[0].x, q[0].y), pack_bf2(q[1].x, q[1].y),
-                    pack_bf2(q[2].x, q[2].y), pack_bf2(q[3].x, q[3].y));
-  for (int j = 1; j < cyq; ++j) o[j] = make_uint4(0u, 0u, 0u, 0u);
+// Index of the fine sample (iz, iy, 0) of coarse voxel v, and the fine row and
+// plane strides.
+struct S2dMap {
+  long long o0, W, HW;
+};
+
+template <int F>
+__device__ inline S2dMap s2d_map(long long v, int hc, int wc) {
+  const int xx = (int)(v % wc);
+  const long long r = v / wc;  // (n d + z) h + y of the coarse voxel
+  const int yy = (int)(r % hc);
+  const long long nz = r / hc;
+  S2dMap m;
+  m.W = (long long)F * wc;
+  m.HW = (long long)F * hc * m.W;
+  m.o0 = (F * nz) * m.HW + (long long)F * yy * m.W + (long long)F * xx;
+  return m;
 }
 
-// Depth-to-space output head: for half-resolution voxel v, eight outputs
-// s_k = b[k] + sum_c w[k][c] h[v][c] land on its 2x2x2 block k = 4 iz + 2 iy + ix
-// of the full-resolution volume: out = in + s.  C == 32; w, b staged in LDS.
-template <int C>
-__global__ __launch_bounds__(256) void head_shuffle_kernel(const uint4* h, const float* w,
-                                                           const float* b, const float* in,
-                                                           float* out, int h2, int w2,
-                                                           long long nvox) {
-  __shared__ float ws[8 * C + 8];
-  for (int i = threadIdx.x; i < 8 * C; i += 256) ws[i] = w[i];
-  if (threadIdx.x < 8) ws[8 * C + threadIdx.x] = b[threadIdx.x];
-  __syncthreads();
+template <int F>
+__device__ inline void row_load(const float* p, float (&r)[F]) {
+  if constexpr (F == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    r[0] = t.x;
+    r[1] = t.y;
+  } else {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    r[0] = t.x;
+    r[1] = t.y;
+    r[2] = t.z;
+    r[3] = t.w;
+  }
+}
+
+template <int F>
+__device__ inline void row_store(float* p, const float (&r)[F]) {
+  if constexpr (F == 2) *reinterpret_cast<float2*>(p) = make_float2(r[0], r[1]);
+  else *reinterpret_cast<float4*>(p) = make_float4(r[0], r[1], r[2], r[3]);
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void unshuffle_kernel(const float* x, uint4* y, int hc, int wc,
+                                                        int cyq, long long nvox) {
+  constexpr int K = F * F * F;
   const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
   if (v >= nvox) return;
-  float hv[C];
+  const S2dMap mp = s2d_map<F>(v, hc, wc);
+  float q[K];
+#pragma unroll
+  for (int r = 0; r < F * F; ++r) {  // r = iz F + iy
+    float row[F];
+    row_load<F>(x + mp.o0 + (r / F) * mp.HW + (r % F) * mp.W, row);
+#pragma unroll
+    for (int i = 0; i < F; ++i) q[r * F + i] = row[i];
+  }
+  uint4* o = y + v * cyq;
+#pragma unroll
+  for (int j = 0; j < K / 8; ++j)
+    o[j] = make_uint4(pack_bf2(q[8 * j], q[8 * j + 1]), pack_bf2(q[8 * j + 2], q[8 * j + 3]),
+                      pack_bf2(q[8 * j + 4], q[8 * j + 5]), pack_bf2(q[8 * j + 6], q[8 * j + 7]));
+  for (int j = K / 8; j < cyq; ++j) o[j] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+template <int C>
+__device__ inline void load_bf_row(const uint4* h, long long v, float (&hv)[C]) {
 #pragma unroll
   for (int q = 0; q < C / 8; ++q) {
     const uint4 t = h[v * (C / 8) + q];
@@ -963,101 +1001,114 @@ __global__ __launch_bounds__(256) void head_shuffle_kernel(const uint4* h, const
       hv[8 * q + 2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
     }
   }
-  float s[8];
+}
+
+// Depth-to-space output head: for coarse voxel v, K = F^3 outputs
+// s_k = b[k] + sum_c w[k][c] h[v][c] land on its block, k = (iz F + iy) F + ix:
+// out = in + s.  C == 32; w, b staged in LDS.
+template <int F, int C>
+__global__ __launch_bounds__(256) void head_shuffle_kernel(const uint4* h, const float* w,
+                                                           const float* b, const float* in,
+                                                           float* out, int hc, int wc,
+                                                           long long nvox) {
+  constexpr int K = F * F * F;
+  __shared__ float ws[K * C + K];
+  for (int i = threadIdx.x; i < K * C; i += 256) ws[i] = w[i];
+  if (threadIdx.x < K) ws[K * C + threadIdx.x] = b[threadIdx.x];
+  __syncthreads();
+  const long long v = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nvox) return;
+  float hv[C];
+  load_bf_row<C>(h, v, hv);
+  const S2dMap mp = s2d_map<F>(v, hc, wc);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    float a = ws[8 * C + k];
+  for (int r = 0; r < F * F; ++r) {
+    const long long o = mp.o0 + (r / F) * mp.HW + (r % F) * mp.W;
+    float row[F];
+    row_load<F>(in + o, row);
 #pragma unroll
-    for (int c = 0; c < C; ++c) a = fmaf(ws[k * C + c], hv[c], a);
-    s[k] = a;
-  }
-  const int xx = (int)(v % w2);
-  const long long r = v / w2;
-  const int yy = (int)(r % h2);
-  const long long nz = r / h2;
-  const long long W = 2LL * w2, H = 2LL * h2;
-  const long long o0 = ((2 * nz) * H + 2 * yy) * W + 2 * xx;
+    for (int i = 0; i < F; ++i) {
+      const int k = r * F + i;
+      float acc = ws[K * C + k];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const long long o = o0 + ((k >> 1) * H + (k & 1)) * W;
-    const float2 i2 = *reinterpret_cast<const float2*>(in + o);
-    *reinterpret_cast<float2*>(out + o) = make_float2(i2.x + s[2 * k], i2.y + s[2 * k + 1]);
+      for (int c = 0; c < C; ++c) acc = fmaf(ws[k * C + c], hv[c], acc);
+      row[i] += acc;
+    }
+    row_store<F>(out + o, row);
   }
 }
 
 // Adjoint of head_shuffle_kernel: dh[v][c] = sum_k w[k][c] g_k (BF16),
-// dw[k][c] += sum_v g_k h[v][c], db[k] += sum_v g_k, with g_k = dout at block k
-// of voxel v.  A block walks tiles of 256 voxels; each tile's g and h go through
-// LDS so thread (k, c) = (t / C, t % C) accumulates its dw entry.  C == 32.
-template <int C>
+// dw[k][c] += sum_v g_k h[v][c], db[k] += sum_v g_k, with g_k = dout at block
+// entry k of voxel v.  A block walks tiles of TV voxels; each tile's g and h go
+// through LDS and thread t accumulates dw[k][t % C] for k = t / C + 8 e,
+// e < K / 8.  C == 32.
+template <int F, int C>
 __global__ __launch_bounds__(256) void head_shuffle_bwd_kernel(const float* dout, const uint4* h,
                                                                const float* w, uint4* dh,
-                                                               float* dw, float* db, int h2,
-                                                               int w2, long long nvox) {
-  static_assert(8 * C == 256, "one dw entry per thread");
-  __shared__ float ws[8 * C];
-  __shared__ float gs[256][9];
-  __shared__ float hs[256][C + 1];
-  for (int i = threadIdx.x; i < 8 * C; i += 256) ws[i] = w[i];
+                                                               float* dw, float* db, int hc,
+                                                               int wc, long long nvox) {
+  constexpr int K = F * F * F, E = K / 8, TV = F == 2 ? 256 : 64;
+  static_assert(8 * C == 256, "8 k-groups of C threads");
+  __shared__ float ws[K * C];
+  __shared__ float gs[TV][K + 1];
+  __shared__ float hs[TV][C + 1];
+  for (int i = threadIdx.x; i < K * C; i += 256) ws[i] = w[i];
   __syncthreads();
-  const int t = threadIdx.x, kk = t / C, cc = t % C;
-  const long long W = 2LL * w2, H = 2LL * h2;
-  float acc = 0.f, accb = 0.f;
-  for (long long tile = (long long)blockIdx.x * 256; tile < nvox; tile += (long long)gridDim.x * 256) {
-    const long long v = tile + t;
-    float g[8], hv[C];
+  const int t = threadIdx.x, kg = t / C, cc = t % C;
+  float acc[E], accb = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) g[k] = 0.f;
+  for (int e = 0; e < E; ++e) acc[e] = 0.f;
+  for (long long tile = (long long)blockIdx.x * TV; tile < nvox; tile += (long long)gridDim.x * TV) {
+    if (t < TV) {
+      const long long v = tile + t;
+      float g[K], hv[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) hv[c] = 0.f;
-    if (v < nvox) {
-      const int xx = (int)(v % w2);
-      const long long r = v / w2;
-      const int yy = (int)(r % h2);
-      const long long nz = r / h2;
-      const long long o0 = ((2 * nz) * H + 2 * yy) * W + 2 * xx;
+      for (int k = 0; k < K; ++k) g[k] = 0.f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float2 g2 = *reinterpret_cast<const float2*>(dout + o0 + ((k >> 1) * H + (k & 1)) * W);
-        g[2 * k] = g2.x;
-        g[2 * k + 1] = g2.y;
-      }
+      for (int c = 0; c < C; ++c) hv[c] = 0.f;
+      if (v < nvox) {
+        const S2dMap mp = s2d_map<F>(v, hc, wc);
 #pragma unroll
-      for (int q = 0; q < C / 8; ++q) {
-        const uint4 u4 = h[v * (C / 8) + q];
-        const unsigned u[4] = {u4.x, u4.y, u4.z, u4.w};
+        for (int r = 0; r < F * F; ++r) {
+          float row[F];
+          row_load<F>(dout + mp.o0 + (r / F) * mp.HW + (r % F) * mp.W, row);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          hv[8 * q + 2 * e] = __uint_as_float(u[e] << 16);
-          hv[8 * q + 2 * e + 1] = __uint_as_float(u[e] & 0xffff0000u);
+          for (int i = 0; i < F; ++i) g[r * F + i] = row[i];
+        }
+        load_bf_row<C>(h, v, hv);
+#pragma unroll
+        for (int q = 0; q < C / 8; ++q) {
+          float d8[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            float a = 0.f;
+#pragma unroll
+            for (int k = 0; k < K; ++k) a = fmaf(ws[k * C + 8 * q + e], g[k], a);
+            d8[e] = a;
+          }
+          dh[v * (C / 8) + q] = make_uint4(pack_bf2(d8[0], d8[1]), pack_bf2(d8[2], d8[3]),
+                                           pack_bf2(d8[4], d8[5]), pack_bf2(d8[6], d8[7]));
         }
       }
 #pragma unroll
-      for (int q = 0; q < C / 8; ++q) {
-        float d8[8];
+      for (int k = 0; k < K; ++k) gs[t][k] = g[k];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          float a = 0.f;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) a = fmaf(ws[k * C + 8 * q + e], g[k], a);
-          d8[e] = a;
-        }
-        dh[v * (C / 8) + q] = make_uint4(pack_bf2(d8[0], d8[1]), pack_bf2(d8[2], d8[3]),
-                                         pack_bf2(d8[4], d8[5]), pack_bf2(d8[6], d8[7]));
-      }
+      for (int c = 0; c < C; ++c) hs[t][c] = hv[c];
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) gs[t][k] = g[k];
-#pragma unroll
-    for (int c = 0; c < C; ++c) hs[t][c] = hv[c];
     __syncthreads();
-    for (int j = 0; j < 256; ++j) acc = fmaf(gs[j][kk], hs[j][cc], acc);
-    if (t < 8)
-      for (int j = 0; j < 256; ++j) accb += gs[j][t];
+    for (int j = 0; j < TV; ++j) {
+      const float hj = hs[j][cc];
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = fmaf(gs[j][kg + 8 * e], hj, acc[e]);
+    }
+    if (t < K)
+      for (int j = 0; j < TV; ++j) accb += gs[j][t];
     __syncthreads();
   }
-  atomicAdd(&dw[t], acc);
-  if (t < 8) atomicAdd(&db[t], accb);
+#pragma unroll
+  for (int e = 0; e < E; ++e) atomicAdd(&dw[(kg + 8 * e) * C + cc], acc[e]);
+  if (t < K) atomicAdd(&db[t], accb);
 }
 
 // Mean-squared error: loss_sum += sum (out - t)^2, dout = 2 (out - t) / nvox.
@@ -1198,6 +1249,7 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   FDK_REQUIRE(cin == 1 || cin % 32 == 0, "cin must be 1 or a multiple of 32 (got %d)", cin);
   FDK_REQUIRE((long long)n * d <= 65535, "n * d exceeds the grid");
   FDK_REQUIRE(cout % 16 == 0, "cout %d not a multiple of 16", cout);
+  FDK_REQUIRE(relu >= 0 && relu <= 2, "activation mode %d not in {0, 1, 2}", relu);
   Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
          cin, cout, relu, out_f32};
   dim3 grid(wd / 32, h / 4, n * d);
@@ -1285,10 +1337,13 @@ extern "C" int fdk_nn_bias_grad(const void* dy, float* db, int64_t nvox, int c, 
   return FDK_OK;
 }
 
-extern "C" int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, void* stream) {
+extern "C" int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, int mode,
+                               void* stream) {
   FDK_REQUIRE(dy && y && dz, "NULL pointer");
+  FDK_REQUIRE(mode == 1 || mode == 2, "activation mode must be 1 (ReLU) or 2 (leaky ReLU)");
   hipLaunchKernelGGL(relu_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const u16*>(dy), static_cast<const u16*>(y), static_cast<u16*>(dz), (long long)n);
+                     static_cast<const u16*>(dy), static_cast<const u16*>(y), static_cast<u16*>(dz), (long long)n,
+                     mode);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -1384,54 +1439,78 @@ extern "C" int fdk_nn_head_bwd(const float* dout, const void* h, const float* w,
   return FDK_OK;
 }
 
-extern "C" int fdk_nn_unshuffle2(const float* x, void* y, int n, int d, int h, int w, int cy,
-                                 void* stream) {
+static bool s2d_sizes_ok(int n, int d, int h, int w, int f) {
+  return (f == 2 || f == 4) && n > 0 && d > 0 && h > 0 && w > 0 && d % f == 0 && h % f == 0 &&
+         w % f == 0;
+}
+
+extern "C" int fdk_nn_unshuffle(const float* x, void* y, int n, int d, int h, int w, int f, int cy,
+                                void* stream) {
   FDK_REQUIRE(x && y, "NULL pointer");
-  FDK_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0 && d % 2 == 0 && h % 2 == 0 && w % 2 == 0,
-              "unshuffle needs even, positive d, h, w (got %d, %d, %d)", d, h, w);
-  FDK_REQUIRE(cy >= 8 && cy % 8 == 0, "cy must be a multiple of 8, at least 8 (got %d)", cy);
-  FDK_REQUIRE(reinterpret_cast<uintptr_t>(x) % 8 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0,
-              "unshuffle needs 8-byte aligned x and 16-byte aligned y");
-  const long long nvox = (long long)n * (d / 2) * (h / 2) * (w / 2);
-  hipLaunchKernelGGL(unshuffle2_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), x, static_cast<uint4*>(y), h / 2, w / 2,
-                     cy / 8, nvox);
+  FDK_REQUIRE(s2d_sizes_ok(n, d, h, w, f),
+              "unshuffle needs f in {2, 4} and d, h, w positive multiples of f (got f %d, %d, %d, %d)",
+              f, d, h, w);
+  FDK_REQUIRE(cy >= f * f * f && cy % 8 == 0, "cy must be a multiple of 8, at least f^3 (got %d)", cy);
+  FDK_REQUIRE(reinterpret_cast<uintptr_t>(x) % (4 * f) == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0,
+              "unshuffle needs 4f-byte aligned x and 16-byte aligned y");
+  const long long nvox = (long long)n * (d / f) * (h / f) * (w / f);
+  const dim3 g((unsigned)((nvox + 255) / 256));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (f == 2)
+    hipLaunchKernelGGL(unshuffle_kernel<2>, g, dim3(256), 0, st, x, static_cast<uint4*>(y), h / 2,
+                       w / 2, cy / 8, nvox);
+  else
+    hipLaunchKernelGGL(unshuffle_kernel<4>, g, dim3(256), 0, st, x, static_cast<uint4*>(y), h / 4,
+                       w / 4, cy / 8, nvox);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
 
 extern "C" int fdk_nn_head_shuffle(const void* h, const float* w, const float* b, const float* in,
-                                   float* out, int n, int d, int hh, int ww, int c, void* stream) {
+                                   float* out, int n, int d, int hh, int ww, int f, int c,
+                                   void* stream) {
   FDK_REQUIRE(h && w && b && in && out, "NULL pointer");
   FDK_REQUIRE(c == 32, "the depth-to-space head takes 32 channels (got %d)", c);
-  FDK_REQUIRE(n > 0 && d > 0 && hh > 0 && ww > 0 && d % 2 == 0 && hh % 2 == 0 && ww % 2 == 0,
-              "head_shuffle needs even, positive d, h, w (got %d, %d, %d)", d, hh, ww);
-  FDK_REQUIRE(reinterpret_cast<uintptr_t>(h) % 16 == 0 && reinterpret_cast<uintptr_t>(in) % 8 == 0 &&
-                  reinterpret_cast<uintptr_t>(out) % 8 == 0,
-              "head_shuffle needs 16-byte aligned h and 8-byte aligned in / out");
-  const long long nvox = (long long)n * (d / 2) * (hh / 2) * (ww / 2);
-  hipLaunchKernelGGL(head_shuffle_kernel<32>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), static_cast<const uint4*>(h), w, b, in, out,
-                     hh / 2, ww / 2, nvox);
+  FDK_REQUIRE(s2d_sizes_ok(n, d, hh, ww, f),
+              "head_shuffle needs f in {2, 4} and d, h, w positive multiples of f (got f %d, %d, %d, %d)",
+              f, d, hh, ww);
+  FDK_REQUIRE(reinterpret_cast<uintptr_t>(h) % 16 == 0 && reinterpret_cast<uintptr_t>(in) % (4 * f) == 0 &&
+                  reinterpret_cast<uintptr_t>(out) % (4 * f) == 0,
+              "head_shuffle needs 16-byte aligned h and 4f-byte aligned in / out");
+  const long long nvox = (long long)n * (d / f) * (hh / f) * (ww / f);
+  const dim3 g((unsigned)((nvox + 255) / 256));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (f == 2)
+    hipLaunchKernelGGL((head_shuffle_kernel<2, 32>), g, dim3(256), 0, st,
+                       static_cast<const uint4*>(h), w, b, in, out, hh / 2, ww / 2, nvox);
+  else
+    hipLaunchKernelGGL((head_shuffle_kernel<4, 32>), g, dim3(256), 0, st,
+                       static_cast<const uint4*>(h), w, b, in, out, hh / 4, ww / 4, nvox);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
 
 extern "C" int fdk_nn_head_shuffle_bwd(const float* dout, const void* h, const float* w, void* dh,
-                                       float* dw, float* db, int n, int d, int hh, int ww, int c,
-                                       void* stream) {
+                                       float* dw, float* db, int n, int d, int hh, int ww, int f,
+                                       int c, void* stream) {
   FDK_REQUIRE(dout && h && w && dh && dw && db, "NULL pointer");
   FDK_REQUIRE(c == 32, "the depth-to-space head takes 32 channels (got %d)", c);
-  FDK_REQUIRE(n > 0 && d > 0 && hh > 0 && ww > 0 && d % 2 == 0 && hh % 2 == 0 && ww % 2 == 0,
-              "head_shuffle needs even, positive d, h, w (got %d, %d, %d)", d, hh, ww);
+  FDK_REQUIRE(s2d_sizes_ok(n, d, hh, ww, f),
+              "head_shuffle needs f in {2, 4} and d, h, w positive multiples of f (got f %d, %d, %d, %d)",
+              f, d, hh, ww);
   FDK_REQUIRE(reinterpret_cast<uintptr_t>(h) % 16 == 0 && reinterpret_cast<uintptr_t>(dh) % 16 == 0 &&
-                  reinterpret_cast<uintptr_t>(dout) % 8 == 0,
-              "head_shuffle_bwd needs 16-byte aligned h, dh and 8-byte aligned dout");
-  const long long nvox = (long long)n * (d / 2) * (hh / 2) * (ww / 2);
-  hipLaunchKernelGGL(head_shuffle_bwd_kernel<32>, dim3(std::min<long long>(1024, (nvox + 255) / 256)),
-                     dim3(256), 0, static_cast<hipStream_t>(stream), dout,
-                     static_cast<const uint4*>(h), w, static_cast<uint4*>(dh), dw, db, hh / 2,
-                     ww / 2, nvox);
+                  reinterpret_cast<uintptr_t>(dout) % (4 * f) == 0,
+              "head_shuffle_bwd needs 16-byte aligned h, dh and 4f-byte aligned dout");
+  const long long nvox = (long long)n * (d / f) * (hh / f) * (ww / f);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (f == 2)
+    hipLaunchKernelGGL((head_shuffle_bwd_kernel<2, 32>), dim3(std::min<long long>(1024, (nvox + 255) / 256)),
+                       dim3(256), 0, st, dout, static_cast<const uint4*>(h), w,
+                       static_cast<uint4*>(dh), dw, db, hh / 2, ww / 2, nvox);
+  else
+    hipLaunchKernelGGL((head_shuffle_bwd_kernel<4, 32>), dim3(std::min<long long>(1024, (nvox + 63) / 64)),
+                       dim3(256), 0, st, dout, static_cast<const uint4*>(h), w,
+                       static_cast<uint4*>(dh), dw, db, hh / 4, ww / 4, nvox);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

@@ -10,8 +10,8 @@ Pipeline pieces:
                       photon noise (I0 = 1e5), GPU FDK from all or every k-th view
                       into any sub-box.
 * ``make_pairs``      training pairs (FDK from 1/4 of the views, FDK from all)
-                      cut into cubic patches (128^3; 256^3 for the
-                      space-to-depth network).
+                      cut into patches (128^3; larger for the space-to-depth
+                      networks, whose coarsest level needs 32 voxels in x).
 * ``train``           Adam on batches of patches, all in the library's kernels.
 * ``infer_tiled``     cubic tiles with a 16-voxel overlap over a whole volume;
                       each tile contributes its centre, borders at the edges.
@@ -108,16 +108,23 @@ class Scanner:
         return vol
 
 
-def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4, rng=None,
-               patch: int = PATCH):
+def _dhw(v):
+    return tuple(int(a) for a in v) if isinstance(v, (tuple, list)) else (int(v),) * 3
+
+
+def make_pairs(scanner: Scanner, seeds, region=256, sparse_step: int = 4, rng=None,
+               patch=PATCH):
     """For each seed: a noisy scan, FDK from every sparse_step-th view and from all
-    views inside a random region^3 box, cut into (region/patch)^3 patches.
-    Returns (inputs, targets), each a (P, patch, patch, patch) FP32 device tensor."""
+    views inside a random box of `region` voxels (an edge, or (D, H, W)), cut into
+    patches of `patch` voxels (likewise).  Returns (inputs, targets), each a
+    (P, pd, ph, pw) FP32 device tensor."""
     t = scanner.torch
     p = scanner.p
     rng = np.random.default_rng(0) if rng is None else rng
-    per = (region // patch) ** 3
-    xs = t.empty((len(seeds) * per, patch, patch, patch), device="cuda")
+    rd, rh, rw = _dhw(region)
+    pd, ph, pw = _dhw(patch)
+    per = (rd // pd) * (rh // ph) * (rw // pw)
+    xs = t.empty((len(seeds) * per, pd, ph, pw), device="cuda")
     ys = t.empty_like(xs)
     raw = t.empty((p.nproj, p.nv, p.nu), device="cuda")
     store = t.empty((p.nproj, p.nu, p.nv), device="cuda")
@@ -125,26 +132,28 @@ def make_pairs(scanner: Scanner, seeds, region: int = 256, sparse_step: int = 4,
     for seed in seeds:
         ell = scanner.phantom(seed)
         scanner.project(ell, noise_seed=seed + 7919, out=raw)
-        lo, hi = p.nx // 8, p.nx - p.nx // 8 - region
-        org = [int(rng.integers(lo, max(lo + 1, hi))) for _ in range(3)]
-        box = (org[0], org[1], org[2], region, region, region)
+        org = [int(rng.integers(p.nx // 8, max(p.nx // 8 + 1, p.nx - p.nx // 8 - r)))
+               for r in (rw, rh, rd)]
+        box = (org[0], org[1], org[2], rw, rh, rd)
         sparse = scanner.fdk(raw, sparse_step, box, store)
         full = scanner.fdk(raw, 1, box, store)
-        for a in range(0, region, patch):
-            for b in range(0, region, patch):
-                for c in range(0, region, patch):
-                    box_copy(sparse, (a, b, c), xs[q], (0, 0, 0), (patch, patch, patch))
-                    box_copy(full, (a, b, c), ys[q], (0, 0, 0), (patch, patch, patch))
+        for a in range(0, rd, pd):
+            for b in range(0, rh, ph):
+                for c in range(0, rw, pw):
+                    box_copy(sparse, (a, b, c), xs[q], (0, 0, 0), (pd, ph, pw))
+                    box_copy(full, (a, b, c), ys[q], (0, 0, 0), (pd, ph, pw))
                     q += 1
     t.cuda.synchronize()
     return xs, ys
 
 
 def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
-          log=None, max_seconds: float | None = None, lr_final: float = 0.05):
+          log=None, max_seconds: float | None = None, lr_final: float = 0.05,
+          warmup: float = 0.0):
     """Adam on random batches of patch pairs (stops early after max_seconds),
     the learning rate following a cosine from lr to lr * lr_final over the run
-    (its progress is the larger of the step and time fractions).  Returns the
+    (its progress is the larger of the step and time fractions), after a linear
+    ramp from lr * lr_final over the first `warmup` of the run.  Returns the
     per-step mean losses."""
     import math
     import torch
@@ -163,6 +172,8 @@ def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, see
                 break
         progress = min(max(s / max(steps - 1, 1), progress_t), 1.0)
         lr_s = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * progress)))
+        if progress < warmup:
+            lr_s *= lr_final + (1.0 - lr_final) * progress / warmup
         idx = torch.as_tensor(rng.choice(xs.shape[0], size=batch, replace=False)).cuda()
         x = xs.index_select(0, idx)
         y = ys.index_select(0, idx)
@@ -186,8 +197,8 @@ def tiles(n: int, patch: int = PATCH, overlap: int = OVERLAP):
 
 
 def tile_size(net: UNet3D) -> int:
-    """The cubic tile the network takes: 128, or 256 for the space-to-depth form."""
-    return 2 * PATCH if net.shuffle else PATCH
+    """The smallest cubic tile the network takes: 128 x its space-to-depth edge."""
+    return PATCH * net.s2d
 
 
 def infer_tiled(net: UNet3D, vol, out=None, batch: int = 4):
@@ -217,11 +228,11 @@ def infer_slabs(net: UNet3D, vol, out=None, slab: int | None = None, overlap: in
     split at the midpoint), batch 1: 1.125x redundant work at 1024^3 with
     128-deep slabs against 1.43x for 128^3 tiles.  slab=None: 128 for the
     full-resolution network, the whole volume in one pass (no redundant work)
-    for the space-to-depth network.  D >= slab; H, W as UNet3D requires."""
+    for the space-to-depth networks.  D >= slab; H, W as UNet3D requires."""
     import torch
     D = vol.shape[0]
     if slab is None:
-        slab = D if net.shuffle else PATCH
+        slab = D if net.s2d > 1 else PATCH
     out = torch.empty_like(vol) if out is None else out
     if slab >= D:
         net.forward(vol.unsqueeze(0), out=out.unsqueeze(0))
@@ -257,10 +268,11 @@ def ssim_slices(a, b) -> float:
     return float(np.mean(vals))
 
 
-def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=None):
+def compare_many(nets: dict, scanner: Scanner, seed: int, sparse_step: int = 4, log=None):
     """Whole-volume comparison on one held-out phantom: plain FDK from all views
-    vs FDK from every sparse_step-th view, with and without the network.
-    Times are GPU wall times with the noisy projections already in HBM."""
+    vs FDK from every sparse_step-th view, with and without each network of
+    `nets` (name -> UNet3D).  Times are GPU wall times with the noisy projections
+    already in HBM; each network is warmed once on the same volume first."""
     import torch
     p = scanner.p
     ell = scanner.phantom(seed)
@@ -268,9 +280,7 @@ def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=
     store = torch.empty((p.nproj, p.nu, p.nv), device="cuda")
     scanner.tables(p.nproj)
     scanner.tables(p.nproj // sparse_step)
-    # warm the kernels once on a small box
-    scanner.fdk(raw, 1, (0, 0, 0, 64, 64, 64), store)
-    net.forward(torch.zeros((1,) + tuple(net.min_shape), device="cuda"))
+    scanner.fdk(raw, 1, (0, 0, 0, 64, 64, 64), store)  # warm the FDK kernels on a small box
     torch.cuda.synchronize()
 
     t0 = time.perf_counter()
@@ -282,26 +292,50 @@ def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=
     sparse = scanner.fdk(raw, sparse_step, None, store)
     torch.cuda.synchronize()
     t_sparse = time.perf_counter() - t0
-    infer_slabs(net, sparse)  # warm: code objects and the allocator's blocks for these shapes
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    learned = infer_slabs(net, sparse)
-    torch.cuda.synchronize()
-    t_net = time.perf_counter() - t0
+    del raw, store
 
     phantom = torch.empty_like(full)
     phantom_rasterize(ell, p, phantom)
     torch.cuda.synchronize()
     res = {
         "volume": [p.nz, p.ny, p.nx], "views_full": p.nproj, "views_sparse": p.nproj // sparse_step,
-        "t_full_fdk_s": t_full, "t_sparse_fdk_s": t_sparse, "t_network_s": t_net,
-        "t_learned_s": t_sparse + t_net, "speedup_vs_full_fdk": t_full / (t_sparse + t_net),
-        "rmse_vs_full_fdk": {"sparse_fdk": rmse(sparse, full), "sparse_fdk+net": rmse(learned, full)},
-        "rmse_vs_phantom": {"full_fdk": rmse(full, phantom), "sparse_fdk": rmse(sparse, phantom),
-                            "sparse_fdk+net": rmse(learned, phantom)},
-        "ssim_vs_full_fdk": {"sparse_fdk": ssim_slices(sparse, full),
-                             "sparse_fdk+net": ssim_slices(learned, full)},
+        "t_full_fdk_s": t_full, "t_sparse_fdk_s": t_sparse,
+        "rmse_vs_full_fdk": {"sparse_fdk": rmse(sparse, full)},
+        "rmse_vs_phantom": {"full_fdk": rmse(full, phantom), "sparse_fdk": rmse(sparse, phantom)},
+        "ssim_vs_full_fdk": {"sparse_fdk": ssim_slices(sparse, full)},
+        "networks": {},
     }
+    learned = torch.empty_like(sparse)
+    for name, net in nets.items():
+        infer_slabs(net, sparse, learned)  # warm: code objects and the allocator's blocks
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infer_slabs(net, sparse, learned)
+        torch.cuda.synchronize()
+        t_net = time.perf_counter() - t0
+        res["networks"][name] = {
+            "s2d": net.s2d, "t_network_s": t_net, "t_learned_s": t_sparse + t_net,
+            "speedup_vs_full_fdk": t_full / (t_sparse + t_net),
+            "rmse_vs_full_fdk": rmse(learned, full), "rmse_vs_phantom": rmse(learned, phantom),
+            "ssim_vs_full_fdk": ssim_slices(learned, full),
+        }
+        torch.cuda.empty_cache()
     if log:
         log(str(res))
     return res
+
+
+def compare(net: UNet3D, scanner: Scanner, seed: int, sparse_step: int = 4, log=None):
+    """compare_many for one network, flattened: t_network_s, t_learned_s,
+    speedup_vs_full_fdk and the 'sparse_fdk+net' entries of the RMSE / SSIM
+    dictionaries."""
+    r = compare_many({"net": net}, scanner, seed, sparse_step)
+    n = r.pop("networks")["net"]
+    r["t_network_s"], r["t_learned_s"] = n["t_network_s"], n["t_learned_s"]
+    r["speedup_vs_full_fdk"] = n["speedup_vs_full_fdk"]
+    r["rmse_vs_full_fdk"]["sparse_fdk+net"] = n["rmse_vs_full_fdk"]
+    r["rmse_vs_phantom"]["sparse_fdk+net"] = n["rmse_vs_phantom"]
+    r["ssim_vs_full_fdk"]["sparse_fdk+net"] = n["ssim_vs_full_fdk"]
+    if log:
+        log(str(r))
+    return r

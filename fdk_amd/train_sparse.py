@@ -1,13 +1,13 @@
 """Train the sparse-view U-Net on the 1024^3 / 1024-view scan and evaluate it.
 
     python -m fdk_amd.train_sparse --phantoms 200 --steps 600 --out fdk_amd/weights/unet_sparse1024.npz
-    python -m fdk_amd.train_sparse --shuffle --phantoms 480 --max-seconds 900 \
-        --out fdk_amd/weights/unet_sparse1024_s2d.npz
+    python -m fdk_amd.train_sparse --s2d 2 --phantoms 480 --max-seconds 900 \
+        --out fdk_amd/weights/unet_sparse1024_s2d2.npz
 
 Pairs: FDK from every 4th of 1024 noisy views (I0 = 1e5) and FDK from all 1024,
-over random 256^3 boxes of seeded ellipsoid phantoms, cut into 128^3 patches
-(whole 256^3 boxes for the space-to-depth network, ``--shuffle``).  Adam (lr
-1e-3), batch 4.  The held-out comparison (``--eval-seeds``) runs on the whole
+over random boxes of seeded ellipsoid phantoms: 256^3 boxes cut into 128^3
+patches, or whole 256^3 (``--s2d 2``) or 256 x 256 x 512 (``--s2d 4``) boxes for
+the space-to-depth networks.  Adam (lr 1e-3), batch 4.  The held-out comparison (``--eval-seeds``) runs on the whole
 1024^3 volume.  Everything runs on the GPU in the library's kernels.
 """
 from __future__ import annotations
@@ -31,10 +31,14 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--max-seconds", type=float, default=None, help="training time budget")
     ap.add_argument("--init", default=None, help="start from saved weights")
-    ap.add_argument("--shuffle", action="store_true",
-                    help="space-to-depth network (runs on the half-resolution grid)")
-    ap.add_argument("--patch", type=int, default=None,
-                    help="training patch edge (default 128, 256 with --shuffle)")
+    ap.add_argument("--s2d", type=int, default=1, choices=(1, 2, 4),
+                    help="space-to-depth block edge (the network runs on a grid that much coarser)")
+    ap.add_argument("--act", type=int, default=1, choices=(1, 2), help="1 ReLU, 2 leaky ReLU")
+    ap.add_argument("--warmup", type=float, default=0.0,
+                    help="fraction of the run over which the learning rate ramps up linearly")
+    ap.add_argument("--patch", default=None,
+                    help="training patch, an edge or D,H,W (default 128; 256 for --s2d 2; "
+                         "256,256,512 for --s2d 4)")
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[100000])
     ap.add_argument("--report", default=None, help="write the evaluation as JSON")
     a = ap.parse_args(argv)
@@ -49,26 +53,31 @@ def main(argv=None):
 
     p = Lr.geometry(a.n, a.views)
     sc = Lr.Scanner(p)
-    patch = a.patch or (256 if a.shuffle else Lr.PATCH)
+    if a.patch:
+        patch = tuple(int(v) for v in a.patch.split(","))
+        patch = patch * 3 if len(patch) == 1 else patch
+    else:
+        patch = {1: (Lr.PATCH,) * 3, 2: (256,) * 3, 4: (256, 256, 512)}[a.s2d]
     if a.init:
         net = UNet3D.from_file(a.init)
-        if net.shuffle != a.shuffle:
-            raise SystemExit(f"{a.init} holds shuffle={net.shuffle} weights")
+        if (net.s2d, net.act) != (a.s2d, a.act):
+            raise SystemExit(f"{a.init} holds s2d={net.s2d}, act={net.act} weights")
     else:
-        net = UNet3D(base=32, seed=0, shuffle=a.shuffle)
+        net = UNet3D(base=32, seed=0, s2d=a.s2d, act=a.act)
     report = {"geometry": {"n": a.n, "views": a.views, "voxel_mm": p.dx, "pixel_mm": p.du}}
     if a.steps > 0:
         t0 = time.perf_counter()
-        xs, ys = Lr.make_pairs(sc, list(range(a.phantoms)), region=max(a.region, patch),
+        region = patch if a.s2d > 1 else (a.region,) * 3
+        xs, ys = Lr.make_pairs(sc, list(range(a.phantoms)), region=region,
                                rng=np.random.default_rng(1), patch=patch)
         log(f"{xs.shape[0]} patch pairs from {a.phantoms} phantoms in {time.perf_counter() - t0:.1f} s")
         t0 = time.perf_counter()
         losses = Lr.train(net, xs, ys, a.steps, a.batch, a.lr, seed=2, log=log,
-                          max_seconds=a.max_seconds)
+                          max_seconds=a.max_seconds, warmup=a.warmup)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         log(f"{len(losses)} steps in {dt:.1f} s ({dt / len(losses) * 1e3:.1f} ms/step)")
-        report["training"] = {"pairs": int(xs.shape[0]), "patch": patch, "shuffle": a.shuffle,
+        report["training"] = {"pairs": int(xs.shape[0]), "patch": list(patch), "s2d": a.s2d, "act": a.act, "warmup": a.warmup,
                               "steps": len(losses), "batch": a.batch,
                               "lr": a.lr, "seconds": dt, "mse_first": float(np.mean(losses[:10])),
                               "mse_last": float(np.mean(losses[-10:]))}

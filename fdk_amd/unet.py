@@ -4,11 +4,11 @@ Architecture: 3 levels, base 32 channels, two 3x3x3 conv + ReLU per level,
 2x2x2 average pooling down, nearest 2x upsampling up with channel-concatenated
 skips, a 1x1 output head, and a residual connection (output = input + net).
 
-With ``shuffle=True`` the network runs on the half-resolution grid: the input's
-2x2x2 voxel blocks become 8 channels (space-to-depth, zero-padded to 32 for the
-first convolution) and an 8-output head writes each block back
+With ``s2d=f`` (2 or 4) the network runs on a coarse grid: the input's f^3
+voxel blocks become f^3 channels (space-to-depth; 8 channels are zero-padded to
+32 for the first convolution) and an f^3-output head writes each block back
 (depth-to-space).  Nothing of the input is lost, and every convolution runs on
-1/8 of the voxels.
+1/f^3 of the voxels.
 
 Every operation runs in the library's HIP kernels (fdk_nn_*): BF16 MFMA
 convolutions with FP32 accumulation, BF16 channels-last activations, FP32
@@ -49,16 +49,16 @@ def _sig():
     L.fdk_nn_conv3d_wgrad.argtypes = [vp, vp, vp, i, i, i, i, i, i, vp]
     L.fdk_nn_flip_weights.argtypes = [vp, vp, i, i, vp]
     L.fdk_nn_bias_grad.argtypes = [vp, vp, i64, i, vp]
-    L.fdk_nn_relu_bwd.argtypes = [vp, vp, vp, i64, vp]
+    L.fdk_nn_relu_bwd.argtypes = [vp, vp, vp, i64, i, vp]
     L.fdk_nn_avgpool2.argtypes = [vp, vp, i, i, i, i, i, i, vp]
     L.fdk_nn_upsample2.argtypes = [vp, vp, i, i, i, i, i, i, i, i, vp]
     L.fdk_nn_channel_copy.argtypes = [vp, i, i, vp, i, i, i, i64, vp]
     L.fdk_nn_add.argtypes = [vp, vp, vp, i64, vp]
     L.fdk_nn_head.argtypes = [vp, vp, vp, vp, vp, i64, i, vp]
     L.fdk_nn_head_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i64, i, vp]
-    L.fdk_nn_unshuffle2.argtypes = [vp, vp, i, i, i, i, i, vp]
-    L.fdk_nn_head_shuffle.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, vp]
-    L.fdk_nn_head_shuffle_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, i, i, vp]
+    L.fdk_nn_unshuffle.argtypes = [vp, vp, i, i, i, i, i, i, vp]
+    L.fdk_nn_head_shuffle.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, vp]
+    L.fdk_nn_head_shuffle_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, i, i, i, vp]
     L.fdk_nn_mse.argtypes = [vp, vp, vp, vp, i64, vp]
     L.fdk_nn_adam.argtypes = [vp, vp, vp, vp, vp, i64, f, f, f, f, i, i, i, vp]
     L.fdk_nn_convert.argtypes = [vp, vp, i64, i, vp]
@@ -99,8 +99,8 @@ def bias_grad(dy, db, nvox, c):
     _check(L().fdk_nn_bias_grad(_p(dy), _p(db), nvox, c, _st()), "bias_grad")
 
 
-def relu_bwd(dy, y, dz):
-    _check(L().fdk_nn_relu_bwd(_p(dy), _p(y), _p(dz), dy.numel(), _st()), "relu_bwd")
+def relu_bwd(dy, y, dz, mode=1):
+    _check(L().fdk_nn_relu_bwd(_p(dy), _p(y), _p(dz), dy.numel(), int(mode), _st()), "relu_bwd")
     return dz
 
 
@@ -126,24 +126,24 @@ def add(a, b, y):
     return y
 
 
-def unshuffle2(x32, y, cy=32):
-    """FP32 (N, D, H, W) -> BF16 (N, D/2, H/2, W/2, cy) space-to-depth."""
+def unshuffle(x32, y, f=2):
+    """FP32 (N, D, H, W) -> BF16 (N, D/f, H/f, W/f, y.shape[-1]) space-to-depth."""
     n, d, h, w = x32.shape
-    _check(L().fdk_nn_unshuffle2(_p(x32), _p(y), n, d, h, w, cy, _st()), "unshuffle2")
+    _check(L().fdk_nn_unshuffle(_p(x32), _p(y), n, d, h, w, f, y.shape[-1], _st()), "unshuffle")
     return y
 
 
-def head_shuffle(hid, w, b, x32, out):
+def head_shuffle(hid, w, b, x32, out, f=2):
     n, d, h, wd = x32.shape
-    _check(L().fdk_nn_head_shuffle(_p(hid), _p(w), _p(b), _p(x32), _p(out), n, d, h, wd,
+    _check(L().fdk_nn_head_shuffle(_p(hid), _p(w), _p(b), _p(x32), _p(out), n, d, h, wd, f,
                                    hid.shape[-1], _st()), "head_shuffle")
     return out
 
 
-def head_shuffle_bwd(dout, hid, w, dh, dw, db):
+def head_shuffle_bwd(dout, hid, w, dh, dw, db, f=2):
     n, d, h, wd = dout.shape
     _check(L().fdk_nn_head_shuffle_bwd(_p(dout), _p(hid), _p(w), _p(dh), _p(dw), _p(db), n, d, h,
-                                       wd, hid.shape[-1], _st()), "head_shuffle_bwd")
+                                       wd, f, hid.shape[-1], _st()), "head_shuffle_bwd")
     return dh
 
 
@@ -178,13 +178,14 @@ def poisson_noise(proj, i0, mu, seed):
 
 # -------------------------------------------------------------------- layers
 class Conv3:
-    """3x3x3 'same' convolution (+ bias, optional ReLU) with Adam state."""
+    """3x3x3 'same' convolution (+ bias, activation) with Adam state.  relu: the
+    activation mode, 0 none, 1 ReLU, 2 leaky ReLU."""
 
     def __init__(self, cin, cout, relu, rng, dev, live_cin=None):
         """live_cin: input channels that carry data (the rest are zero padding
         and get zero weights)."""
         torch = _torch()
-        self.cin, self.cout, self.relu = cin, cout, relu
+        self.cin, self.cout, self.relu = cin, cout, int(relu)
         live = cin if live_cin is None else live_cin
         std = np.sqrt(2.0 / (27 * live))
         if cin == 1:
@@ -217,7 +218,7 @@ class Conv3:
     def backward(self, dy, x, y, shape, need_dx=True):
         """dy: gradient at the (post-ReLU) output y; x: the layer input."""
         torch = _torch()
-        dz = relu_bwd(dy, y, torch.empty_like(dy)) if self.relu else dy
+        dz = relu_bwd(dy, y, torch.empty_like(dy), self.relu) if self.relu else dy
         conv3d_wgrad(x, dz, self.gw, shape, self.cin, self.cout)
         bias_grad(dz, self.gb, int(np.prod(shape)), self.cout)
         if not need_dx:
@@ -238,32 +239,39 @@ def adam(w, wbf, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
 
 class UNet3D:
     """Residual U-Net: out = x + head(dec1(...)).  Input / output (N, D, H, W)
-    FP32 tensors; D % 4, H % 16 and W % 128 == 0 (shuffle:
-    D % 8, H % 32 and W % 256 == 0)."""
+    FP32 tensors; D % 4f, H % 16f and W % 128f == 0 for block edge f = s2d
+    (1: no space-to-depth)."""
 
-    def __init__(self, base: int = 32, seed: int = 0, device="cuda", shuffle: bool = False):
+    def __init__(self, base: int = 32, seed: int = 0, device="cuda", s2d: int = 1,
+                 act: int = 1):
+        """act: 1 ReLU, 2 leaky ReLU after every convolution."""
         torch = _torch()
         rng = np.random.default_rng(seed)
         c1, c2, c3 = base, 2 * base, 4 * base
         self.c = (c1, c2, c3)
-        self.shuffle = bool(shuffle)
-        if self.shuffle:
+        if s2d not in (1, 2, 4):
+            raise ValueError(f"s2d must be 1, 2 or 4 (got {s2d})")
+        if act not in (1, 2):
+            raise ValueError(f"act must be 1 (ReLU) or 2 (leaky ReLU) (got {act})")
+        self.s2d, self.act = int(s2d), int(act)
+        if self.s2d > 1:
             assert c1 == 32, "the depth-to-space head takes 32 channels"
-            self.e1a = Conv3(32, c1, True, rng, device, live_cin=8)
+            k = self.s2d ** 3
+            self.e1a = Conv3(max(32, k), c1, act, rng, device, live_cin=k)
         else:
-            self.e1a = Conv3(1, c1, True, rng, device)
-        self.e1b = Conv3(c1, c1, True, rng, device)
-        self.e2a = Conv3(c1, c2, True, rng, device)
-        self.e2b = Conv3(c2, c2, True, rng, device)
-        self.ba = Conv3(c2, c3, True, rng, device)
-        self.bb = Conv3(c3, c3, True, rng, device)
-        self.d2a = Conv3(c3 + c2, c2, True, rng, device)
-        self.d2b = Conv3(c2, c2, True, rng, device)
-        self.d1a = Conv3(c2 + c1, c1, True, rng, device)
-        self.d1b = Conv3(c1, c1, True, rng, device)
-        nh = 8 if self.shuffle else 1
+            self.e1a = Conv3(1, c1, act, rng, device)
+        self.e1b = Conv3(c1, c1, act, rng, device)
+        self.e2a = Conv3(c1, c2, act, rng, device)
+        self.e2b = Conv3(c2, c2, act, rng, device)
+        self.ba = Conv3(c2, c3, act, rng, device)
+        self.bb = Conv3(c3, c3, act, rng, device)
+        self.d2a = Conv3(c3 + c2, c2, act, rng, device)
+        self.d2b = Conv3(c2, c2, act, rng, device)
+        self.d1a = Conv3(c2 + c1, c1, act, rng, device)
+        self.d1b = Conv3(c1, c1, act, rng, device)
+        nh = self.s2d ** 3
         hw = (rng.standard_normal((nh, c1)) * 1e-3).astype(np.float32)
-        self.hw = torch.as_tensor(hw if self.shuffle else hw[0]).to(device)
+        self.hw = torch.as_tensor(hw if self.s2d > 1 else hw[0]).to(device)
         self.hb = torch.zeros(nh, device=device)
         self.ghw = torch.zeros_like(self.hw)
         self.ghb = torch.zeros_like(self.hb)
@@ -296,9 +304,10 @@ class UNet3D:
             if keep:
                 k.update(t)
 
-        if self.shuffle:
-            D, H, W = D // 2, H // 2, W // 2
-            xb = unshuffle2(x32, bf((N, D, H, W), 32))
+        f = self.s2d
+        if f > 1:
+            D, H, W = D // f, H // f, W // f
+            xb = unshuffle(x32, bf((N, D, H, W), self.e1a.cin), f)
         else:
             xb = to_bf16(x32, torch.empty(x32.shape, dtype=I16, device=dev))
         s1_, s2_, s3_ = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
@@ -346,8 +355,8 @@ class UNet3D:
         del d1a
         if out is None:
             out = torch.empty(x32.shape, dtype=torch.float32, device=dev)
-        if self.shuffle:
-            head_shuffle(d1, self.hw, self.hb, x32, out)
+        if f > 1:
+            head_shuffle(d1, self.hw, self.hb, x32, out, f)
         else:
             _check(L().fdk_nn_head(_p(d1), _p(self.hw), _p(self.hb), _p(x32), _p(out),
                                    out.numel(), c1, _st()), "head")
@@ -371,8 +380,8 @@ class UNet3D:
         dev = dout.device
         n1, n2 = int(np.prod(s1_)), int(np.prod(s2_))
         dd1 = torch.empty(s1_ + (c1,), dtype=I16, device=dev)
-        if self.shuffle:
-            head_shuffle_bwd(dout, k["d1"], self.hw, dd1, self.ghw, self.ghb)
+        if self.s2d > 1:
+            head_shuffle_bwd(dout, k["d1"], self.hw, dd1, self.ghw, self.ghb, self.s2d)
         else:
             _check(L().fdk_nn_head_bwd(_p(dout), _p(k["d1"]), _p(self.hw), _p(dd1), _p(self.ghw),
                                        _p(self.ghb), dout.numel(), c1, _st()), "head_bwd")
@@ -423,18 +432,19 @@ class UNet3D:
     def check_shape(self, shape):
         """Raise unless an (N, D, H, W) input fits the network's kernels."""
         _, D, H, W = shape
-        need = (8, 32, 256) if self.shuffle else (4, 16, 128)
+        need = self.min_shape
         if D % need[0] or H % need[1] or W % need[2]:
             raise ValueError(f"U-Net input (D, H, W) = {(D, H, W)} must be multiples of {need}")
 
     @property
     def min_shape(self):
         """The smallest (D, H, W) input."""
-        return (8, 32, 256) if self.shuffle else (4, 16, 128)
+        f = self.s2d
+        return (4 * f, 16 * f, 128 * f)
 
     # ------------------------------------------------------------ persistence
     def state_dict(self):
-        d = {"arch.shuffle": np.array(int(self.shuffle), np.int32)}
+        d = {"arch.s2d": np.array(self.s2d, np.int32), "arch.act": np.array(self.act, np.int32)}
         for i, cv in enumerate(self.convs):
             d[f"conv{i}.w"] = cv.w.cpu().numpy()
             d[f"conv{i}.b"] = cv.b.cpu().numpy()
@@ -444,10 +454,11 @@ class UNet3D:
 
     def load_state_dict(self, d):
         torch = _torch()
-        shuffle = bool(int(d["arch.shuffle"])) if "arch.shuffle" in d else False
-        if shuffle != self.shuffle:
-            raise ValueError(f"weights are for shuffle={shuffle}, the network has "
-                             f"shuffle={self.shuffle}")
+        s2d = int(d["arch.s2d"]) if "arch.s2d" in d else 1
+        act = int(d["arch.act"]) if "arch.act" in d else 1
+        if (s2d, act) != (self.s2d, self.act):
+            raise ValueError(f"weights are for s2d={s2d}, act={act}; the network has "
+                             f"s2d={self.s2d}, act={self.act}")
         for i, cv in enumerate(self.convs):
             cv.w.copy_(torch.as_tensor(d[f"conv{i}.w"]))
             cv.b.copy_(torch.as_tensor(d[f"conv{i}.b"]))
@@ -467,8 +478,9 @@ class UNet3D:
         """A network built for the saved weights' architecture, with them loaded."""
         with np.load(path, allow_pickle=False) as z:
             d = {k: z[k] for k in z.files}
-        shuffle = bool(int(d["arch.shuffle"])) if "arch.shuffle" in d else False
+        s2d = int(d["arch.s2d"]) if "arch.s2d" in d else 1
+        act = int(d["arch.act"]) if "arch.act" in d else 1
         base = int(d["conv1.w"].shape[1])
-        net = cls(base=base, seed=0, device=device, shuffle=shuffle)
+        net = cls(base=base, seed=0, device=device, s2d=s2d, act=act)
         net.load_state_dict(d)
         return net

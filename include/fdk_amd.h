@@ -258,7 +258,8 @@ int fdk_phantom_project(const fdk_ellipsoid* host_ell, int n, const fdk_geometry
  * (uint16 storage); weights BF16 for the MFMA, FP32 masters for Adam.
  * A 3x3x3 'same' convolution with zero padding: weights [27][cout][cin]
  * (tap = 9 dz + 3 dy + dx over -1..1), or [cout][32] taps for cin == 1.
- * y = relu?(conv(x) + bias) + resid, stored BF16 or FP32 (out_f32).
+ * y = act(conv(x) + bias) + resid, stored BF16 or FP32 (out_f32); act (the
+ * `relu` argument): 0 none, 1 ReLU, 2 leaky ReLU (slope 0.01 below zero).
  * Needs W % 32 == 0, H % 4 == 0, cin == 1 or cin % 32 == 0, cout in
  * {32, 64, 96, 128, 192}. */
 int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid, void* y,
@@ -272,7 +273,9 @@ int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, 
  * is then fdk_nn_conv3d_fwd(dy, wb, ..., cin = cout, cout = cin). */
 int fdk_nn_flip_weights(const void* w, void* wb, int cin, int cout, void* stream);
 int fdk_nn_bias_grad(const void* dy, float* db, int64_t nvox, int c, void* stream);
-int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, void* stream);
+/* Activation adjoint at output y: dz = dy where y > 0, else 0 (mode 1, ReLU) or
+ * 0.01 dy (mode 2, leaky ReLU). */
+int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t n, int mode, void* stream);
 /* 2x2x2 average pooling of [n][d][h][w][c] (adjoint 1: [n][d/2][h/2][w/2][c] -> full). */
 int fdk_nn_avgpool2(const void* x, void* y, int n, int d, int h, int w, int c, int adjoint,
                     void* stream);
@@ -288,18 +291,20 @@ int fdk_nn_head(const void* h, const float* w, const float* b, const float* in, 
                 int64_t nvox, int c, void* stream);
 int fdk_nn_head_bwd(const float* dout, const void* h, const float* w, void* dh, float* dw,
                     float* db, int64_t nvox, int c, void* stream);
-/* Space-to-depth input: FP32 x[n][d][h][w] -> BF16 y[n][d/2][h/2][w/2][cy], the
- * 2x2x2 block (2z+iz, 2y+iy, 2x+ix) as channels 4 iz + 2 iy + ix, channels
- * 8..cy-1 zero.  d, h, w even; cy % 8 == 0. */
-int fdk_nn_unshuffle2(const float* x, void* y, int n, int d, int h, int w, int cy, void* stream);
-/* Depth-to-space head over the same block map: for half-resolution voxel v and
- * k = 4 iz + 2 iy + ix, out[block k] = in[block k] + b[k] + sum_c w[k][c] h[v][c]
- * (w [8][c], b [8], FP32; d, h, w the full-resolution sizes, even; c == 32),
- * and its adjoint (dh BF16; dw [8][c], db [8] accumulated). */
+/* Space-to-depth input, block edge f (2 or 4): FP32 x[n][d][h][w] -> BF16
+ * y[n][d/f][h/f][w/f][cy], the f^3 block (f z + iz, f y + iy, f x + ix) as
+ * channels (iz f + iy) f + ix, channels f^3..cy-1 zero.  d, h, w multiples of f;
+ * cy % 8 == 0, cy >= f^3. */
+int fdk_nn_unshuffle(const float* x, void* y, int n, int d, int h, int w, int f, int cy,
+                     void* stream);
+/* Depth-to-space head over the same block map: for coarse voxel v and block entry
+ * k, out[entry k] = in[entry k] + b[k] + sum_c w[k][c] h[v][c] (w [f^3][c], b [f^3],
+ * FP32; d, h, w the fine sizes; c == 32), and its adjoint (dh BF16; dw [f^3][c],
+ * db [f^3] accumulated). */
 int fdk_nn_head_shuffle(const void* h, const float* w, const float* b, const float* in, float* out,
-                        int n, int d, int h_, int w_, int c, void* stream);
+                        int n, int d, int h_, int w_, int f, int c, void* stream);
 int fdk_nn_head_shuffle_bwd(const float* dout, const void* h, const float* w, void* dh, float* dw,
-                            float* db, int n, int d, int h_, int w_, int c, void* stream);
+                            float* db, int n, int d, int h_, int w_, int f, int c, void* stream);
 /* loss_sum += sum (out - t)^2; dout = 2 (out - t) / nvox when dout != NULL. */
 int fdk_nn_mse(const float* out, const float* target, float* dout, float* loss_sum, int64_t nvox,
                void* stream);

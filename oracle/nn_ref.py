@@ -85,30 +85,34 @@ def upsample2_adjoint(dy):
     return dy.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2, c).sum(axis=(2, 4, 6))
 
 
-def unshuffle2(x, cy=8):
-    """Space-to-depth: FP32 [n][2d][2h][2w] -> [n][d][h][w][cy]; channel
-    4 iz + 2 iy + ix holds voxel (2z + iz, 2y + iy, 2x + ix), channels 8.. zero."""
+def unshuffle(x, f=2, cy=None):
+    """Space-to-depth, block edge f: FP32 [n][f d][f h][f w] -> [n][d][h][w][cy];
+    channel (iz f + iy) f + ix holds voxel (f z + iz, f y + iy, f x + ix),
+    channels f^3.. zero (cy defaults to f^3)."""
     n, D, H, W = x.shape
-    b = x.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2).transpose(0, 1, 3, 5, 2, 4, 6)
-    out = np.zeros((n, D // 2, H // 2, W // 2, cy))
-    out[..., :8] = b.reshape(n, D // 2, H // 2, W // 2, 8)
+    k = f ** 3
+    cy = k if cy is None else cy
+    b = x.reshape(n, D // f, f, H // f, f, W // f, f).transpose(0, 1, 3, 5, 2, 4, 6)
+    out = np.zeros((n, D // f, H // f, W // f, cy))
+    out[..., :k] = b.reshape(n, D // f, H // f, W // f, k)
     return out
 
 
-def shuffle2(y):
-    """Depth-to-space, the inverse of unshuffle2 on channels 0..7."""
+def shuffle(y, f=2):
+    """Depth-to-space, the inverse of unshuffle on channels 0..f^3-1."""
     n, d, h, w = y.shape[:4]
-    b = y[..., :8].reshape(n, d, h, w, 2, 2, 2).transpose(0, 1, 4, 2, 5, 3, 6)
-    return b.reshape(n, 2 * d, 2 * h, 2 * w)
+    b = y[..., :f ** 3].reshape(n, d, h, w, f, f, f).transpose(0, 1, 4, 2, 5, 3, 6)
+    return b.reshape(n, f * d, f * h, f * w)
 
 
-def head_shuffle(h, w, b, x):
-    """out = x + shuffle2(h w^T + b): h [n][d][h][w][c], w [8][c], b [8]."""
-    return x + shuffle2(h @ w.T + b)
+def head_shuffle(h, w, b, x, f=2):
+    """out = x + shuffle(h w^T + b): h [n][d][h][w][c], w [f^3][c], b [f^3]."""
+    return x + shuffle(h @ w.T + b, f)
 
 
-def head_shuffle_adjoint(dout, h, w):
+def head_shuffle_adjoint(dout, h, w, f=2):
     """(dh, dw, db) of head_shuffle for the output gradient dout."""
-    g = unshuffle2(dout)
+    k = f ** 3
+    g = unshuffle(dout, f)
     c = h.shape[-1]
-    return g @ w, g.reshape(-1, 8).T @ h.reshape(-1, c), g.reshape(-1, 8).sum(0)
+    return g @ w, g.reshape(-1, k).T @ h.reshape(-1, c), g.reshape(-1, k).sum(0)

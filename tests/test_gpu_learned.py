@@ -45,3 +45,36 @@ def test_trained_network_improves_sparse_fdk_on_held_out_phantom(torch_cuda):
     print(f"  z-slab inference: RMSE {r_slabs:.4f}", flush=True)
     assert r_slabs < 0.75 * r_sparse
     assert abs(r_slabs - r_learned) < 0.1 * r_learned
+
+
+@pytest.mark.parametrize("fname,max_ratio", [("unet_sparse1024_s2d2.npz", 0.75),
+                                             ("unet_sparse1024_s2d4.npz", 0.85)])
+def test_space_to_depth_networks_improve_sparse_fdk(torch_cuda, fname, max_ratio):
+    """The space-to-depth networks (block edge 2 and 4, trained by
+    ``python -m fdk_amd.train_sparse --s2d ...``) on a held-out phantom, over a
+    128 x 512 x 512 box of the 1024^3 scan in one pass: RMSE against the
+    1024-view FDK must drop below max_ratio of the 256-view FDK's, and SSIM rise."""
+    torch = torch_cuda
+    from fdk_amd import learned as Lr
+    from fdk_amd.unet import UNet3D
+    path = os.path.join(os.path.dirname(WEIGHTS), fname)
+    net = UNet3D.from_file(path)
+    p = Lr.geometry(1024, 1024)
+    sc = Lr.Scanner(p)
+    seed = 100002  # not a training seed
+    print(f"learned {fname}: held-out phantom seed {seed}, 512 x 512 x 128 box", flush=True)
+    ell = sc.phantom(seed)
+    raw = sc.project(ell, noise_seed=seed + 7919)
+    box = (256, 256, 448, 512, 512, 128)
+    store = torch.empty((p.nproj, p.nu, p.nv), device="cuda")
+    full = sc.fdk(raw, 1, box, store)
+    sparse = sc.fdk(raw, 4, box, store)
+    learned = Lr.infer_slabs(net, sparse)
+    torch.cuda.synchronize()
+    r_sparse, r_learned = Lr.rmse(sparse, full), Lr.rmse(learned, full)
+    s_sparse, s_learned = Lr.ssim_slices(sparse, full), Lr.ssim_slices(learned, full)
+    print(f"  RMSE vs 1024-view FDK: sparse {r_sparse:.4f} -> learned {r_learned:.4f}; "
+          f"SSIM {s_sparse:.4f} -> {s_learned:.4f}", flush=True)
+    assert np.isfinite(r_learned)
+    assert r_learned < max_ratio * r_sparse
+    assert s_learned > s_sparse

@@ -78,7 +78,11 @@ def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
         y = torch.empty(shape + (cout,), dtype=torch.int16, device="cuda")
         U.conv3d(bf(torch, x), bf(torch, w), torch.as_tensor(b).cuda(), y, shape, cin, cout,
                  relu=True)
-        assert rel(from_bf(y), np.maximum(R.conv3d(x, w, b), 0)) < 4e-3
+        z = R.conv3d(x, w, b)
+        assert rel(from_bf(y), np.maximum(z, 0)) < 4e-3
+        U.conv3d(bf(torch, x), bf(torch, w), torch.as_tensor(b).cuda(), y, shape, cin, cout,
+                 relu=2)  # leaky ReLU
+        assert rel(from_bf(y), np.where(z > 0, z, 0.01 * z)) < 4e-3
         res = rng.standard_normal(shape + (cout,)).astype(np.float32)
         y32 = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
         U.conv3d(bf(torch, x), bf(torch, w), None, y32, shape, cin, cout, relu=False,
@@ -184,33 +188,36 @@ def test_pool_upsample_copy_head_mse_adam(torch_cuda):
     assert np.abs(from_bf(wbf) - R.bf16_round(tw.cpu().numpy())).max() == 0.0
 
 
-@pytest.mark.parametrize("full", [(1, 6, 10, 14), (1, 128, 128, 136)])
-def test_space_to_depth_kernels(torch_cuda, full):
-    """unshuffle2 bit-exact; the 8-output head and its adjoint against the FP64
-    oracle (the second shape runs the adjoint's tile loop more than once)."""
+@pytest.mark.parametrize("f,full", [(2, (1, 6, 10, 14)), (2, (1, 128, 128, 136)),
+                                    (4, (1, 12, 20, 28)), (4, (1, 256, 128, 160))])
+def test_space_to_depth_kernels(torch_cuda, f, full):
+    """unshuffle bit-exact; the f^3-output head and its adjoint against the FP64
+    oracle (the larger shapes run the adjoint's tile loop more than once)."""
     torch = torch_cuda
-    rng = np.random.default_rng(12)
+    rng = np.random.default_rng(12 + f)
     n, D, H, W = full
-    half = (n, D // 2, H // 2, W // 2)
+    k = f ** 3
+    coarse = (n, D // f, H // f, W // f)
     x = rng.standard_normal(full).astype(np.float32)
     xd = torch.as_tensor(x).cuda()
-    y = torch.empty(half + (32,), dtype=torch.int16, device="cuda")
-    U.unshuffle2(xd, y)
-    assert np.array_equal(from_bf(y), R.unshuffle2(R.bf16_round(x), 32))
-    h = R.bf16_round(rng.standard_normal(half + (32,)))
-    w = rng.standard_normal((8, 32)).astype(np.float32)
-    b = rng.standard_normal(8).astype(np.float32)
+    cy = max(32, k)
+    y = torch.empty(coarse + (cy,), dtype=torch.int16, device="cuda")
+    U.unshuffle(xd, y, f)
+    assert np.array_equal(from_bf(y), R.unshuffle(R.bf16_round(x), f, cy))
+    h = R.bf16_round(rng.standard_normal(coarse + (32,)))
+    w = rng.standard_normal((k, 32)).astype(np.float32)
+    b = rng.standard_normal(k).astype(np.float32)
     hd, wd, bd = bf(torch, h), torch.as_tensor(w).cuda(), torch.as_tensor(b).cuda()
     out = torch.empty(full, device="cuda")
-    U.head_shuffle(hd, wd, bd, xd, out)
-    assert rel(out.cpu().numpy(), R.head_shuffle(h, w.astype(np.float64), b, x)) < 1e-6
+    U.head_shuffle(hd, wd, bd, xd, out, f)
+    assert rel(out.cpu().numpy(), R.head_shuffle(h, w.astype(np.float64), b, x, f)) < 1e-6
     dout = rng.standard_normal(full).astype(np.float32)
     dh = torch.empty_like(hd)
-    dw = torch.zeros((8, 32), device="cuda")
-    db = torch.zeros(8, device="cuda")
-    U.head_shuffle_bwd(torch.as_tensor(dout).cuda(), hd, wd, dh, dw, db)
+    dw = torch.zeros((k, 32), device="cuda")
+    db = torch.zeros(k, device="cuda")
+    U.head_shuffle_bwd(torch.as_tensor(dout).cuda(), hd, wd, dh, dw, db, f)
     torch.cuda.synchronize()
-    rdh, rdw, rdb = R.head_shuffle_adjoint(dout.astype(np.float64), h, w.astype(np.float64))
+    rdh, rdw, rdb = R.head_shuffle_adjoint(dout.astype(np.float64), h, w.astype(np.float64), f)
     assert rel(from_bf(dh), rdh) < 4e-3  # BF16 output
     assert rel(dw.cpu().numpy(), rdw) < 1e-5
     assert rel(db.cpu().numpy(), rdb) < 1e-5
@@ -235,11 +242,13 @@ def _torch_unet(net, x):
         w = W(cv)
         b = torch.as_tensor(cv.b.cpu().numpy().astype(np.float64)).requires_grad_(True)
         params[id(cv)] = (w, b)
-        return torch.relu(Fn.conv3d(h, w, b, padding=1))
+        z = Fn.conv3d(h, w, b, padding=1)
+        return torch.relu(z) if net.act == 1 else Fn.leaky_relu(z, 0.01)
 
-    if net.shuffle:
+    f = net.s2d
+    if f > 1:
         xt = torch.as_tensor(np.ascontiguousarray(
-            R.unshuffle2(R.bf16_round(x), 32).transpose(0, 4, 1, 2, 3)))
+            R.unshuffle(R.bf16_round(x), f, net.e1a.cin).transpose(0, 4, 1, 2, 3)))
     else:
         xt = torch.as_tensor(R.bf16_round(x)[:, None]).requires_grad_(False)
     e1 = conv(net.e1b, conv(net.e1a, xt))
@@ -249,25 +258,25 @@ def _torch_unet(net, x):
     d1 = conv(net.d1b, conv(net.d1a, torch.cat([Fn.interpolate(d2, scale_factor=2), e1], 1)))
     hw = torch.as_tensor(net.hw.cpu().numpy().astype(np.float64)).requires_grad_(True)
     hb = torch.as_tensor(net.hb.cpu().numpy().astype(np.float64)).requires_grad_(True)
-    if net.shuffle:  # 8 outputs per half-resolution voxel, block k = 4 iz + 2 iy + ix
+    if f > 1:  # f^3 outputs per coarse voxel, block entry k = (iz f + iy) f + ix
         s = torch.einsum("ncdhw,kc->nkdhw", d1, hw) + hb[None, :, None, None, None]
         n, _, d, h, w = s.shape
-        s = s.reshape(n, 2, 2, 2, d, h, w).permute(0, 4, 1, 5, 2, 6, 3)
-        out = torch.as_tensor(x.astype(np.float64)) + s.reshape(n, 2 * d, 2 * h, 2 * w)
+        s = s.reshape(n, f, f, f, d, h, w).permute(0, 4, 1, 5, 2, 6, 3)
+        out = torch.as_tensor(x.astype(np.float64)) + s.reshape(n, f * d, f * h, f * w)
         return out, params, hw, hb
     out = torch.as_tensor(x[:, None].astype(np.float64)) + \
         torch.einsum("ncdhw,c->ndhw", d1, hw)[:, None] + hb
     return out[:, 0], params, hw, hb
 
 
-@pytest.mark.parametrize("shuffle", [False, True])
-def test_unet_forward_and_gradients_match_torch(torch_cuda, shuffle):
+@pytest.mark.parametrize("s2d,act", [(1, 1), (2, 1), (4, 1), (4, 2)])
+def test_unet_forward_and_gradients_match_torch(torch_cuda, s2d, act):
     torch = torch_cuda
-    net = U.UNet3D(base=32, seed=3, shuffle=shuffle)
+    net = U.UNet3D(base=32, seed=3, s2d=s2d, act=act)
     with torch.no_grad():
         net.hw.mul_(300.0)  # a head large enough that the net's output is not just x
     rng = np.random.default_rng(4)
-    shape = (1, 8, 32, 256) if shuffle else (1, 8, 16, 128)
+    shape = (1, 8, 16, 128) if s2d == 1 else (1,) + net.min_shape
     x = rng.standard_normal(shape).astype(np.float32)
     tgt = rng.standard_normal(shape).astype(np.float32)
     xd = torch.as_tensor(x).cuda()
@@ -301,20 +310,25 @@ def test_unet_forward_and_gradients_match_torch(torch_cuda, shuffle):
     print("weight-gradient cosine similarity per conv:", [round(c, 5) for c in cos])
 
 
-@pytest.mark.parametrize("shuffle", [False, True])
-def test_unet_training_reduces_loss(torch_cuda, shuffle):
+@pytest.mark.parametrize("s2d,act", [(1, 1), (2, 1), (4, 1), (4, 2)])
+def test_unet_training_reduces_loss(torch_cuda, s2d, act):
     torch = torch_cuda
-    net = U.UNet3D(base=32, seed=5, shuffle=shuffle)
+    net = U.UNet3D(base=32, seed=5, s2d=s2d, act=act)
     rng = np.random.default_rng(6)
     # a smooth target and a noisy, blurred input: the net must learn a filter
-    d, h, w = (16, 32, 256) if shuffle else (16, 16, 128)
+    d, h, w = (16, 16, 128) if s2d == 1 else (max(16, net.min_shape[0]),) + net.min_shape[1:]
     z, y, x = np.meshgrid(np.linspace(-1, 1, d), np.linspace(-1, 1, h), np.linspace(-1, 1, w),
                           indexing="ij")
     clean = (np.sin(3 * x) * np.cos(2 * y) + z)[None].astype(np.float32)
     clean = np.concatenate([clean, -clean])
-    noisy = clean + 0.3 * rng.standard_normal(clean.shape).astype(np.float32)
+    if s2d < 4:
+        noisy = clean + 0.3 * rng.standard_normal(clean.shape).astype(np.float32)
+    else:
+        # 32 features per coarse voxel cannot undo 64 independent per-voxel errors:
+        # a smooth corruption instead
+        noisy = clean + (0.6 * np.cos(4 * x) * np.sin(3 * z) + 0.2)[None].astype(np.float32)
     xd, td = torch.as_tensor(noisy).cuda(), torch.as_tensor(clean).cuda()
-    steps = 120 if shuffle else 40  # the half-resolution net starts further from a smoother
+    steps = 120 if s2d > 1 else 40  # the coarse-grid nets start further from a smoother
     losses = [net.train_step(xd, td, lr=1e-3).item() for _ in range(steps)]
     print("training loss:", [round(l, 1) for l in losses[::steps // 5]], "->", round(losses[-1], 1))
     assert losses[-1] < 0.5 * losses[0]

@@ -24,3 +24,30 @@ def test_sparse_view_geometry():
     assert p.du == pytest.approx(0.4) and p.dx == pytest.approx(0.4 / 1.5)
     q = Lr.geometry(256, 64)
     assert q.nu * q.du == pytest.approx(409.6)
+
+
+def test_weight_files_declare_their_architecture():
+    """Each committed weight file loads without pickle and its arrays fit the
+    architecture it declares (space-to-depth edge, activation)."""
+    import os
+
+    import numpy as np
+    root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fdk_amd",
+                        "weights")
+    seen = 0
+    for fname in sorted(os.listdir(root)):
+        if not fname.endswith(".npz"):
+            continue
+        with np.load(os.path.join(root, fname), allow_pickle=False) as z:
+            f = int(z["arch.s2d"]) if "arch.s2d" in z.files else 1
+            act = int(z["arch.act"]) if "arch.act" in z.files else 1
+            assert f in (1, 2, 4) and act in (1, 2)
+            k = f ** 3
+            cin0 = 1 if f == 1 else max(32, k)
+            assert z["conv0.w"].shape == ((32, 32) if f == 1 else (27, 32, cin0))
+            assert z["head.w"].shape == ((32,) if f == 1 else (k, 32))
+            assert z["head.b"].shape == (k,)
+            for i in range(10):
+                assert np.all(np.isfinite(z[f"conv{i}.w"]))
+        seen += 1
+    assert seen >= 2

@@ -76,39 +76,42 @@ def test_bf16_rounding():
     assert np.abs(r - vals).max() <= 2 ** -8 * np.abs(vals).max()
 
 
-def test_space_to_depth_index_map_and_round_trip():
+@pytest.mark.parametrize("f", [2, 4])
+def test_space_to_depth_index_map_and_round_trip(f):
     rng = np.random.default_rng(5)
-    x = rng.standard_normal((2, 4, 6, 8))
-    y = R.unshuffle2(x, 16)
-    assert y.shape == (2, 2, 3, 4, 16) and not y[..., 8:].any()
-    for n, z, yy, xx, iz, iy, ix in np.ndindex(2, 2, 3, 4, 2, 2, 2):
-        assert y[n, z, yy, xx, 4 * iz + 2 * iy + ix] == x[n, 2 * z + iz, 2 * yy + iy, 2 * xx + ix]
-    assert np.array_equal(R.shuffle2(y), x)
-    # a head whose weights pick channel j of h puts h[..., j] on block j
-    h = rng.standard_normal((1, 2, 3, 4, 32))
-    w = np.zeros((8, 32))
-    w[np.arange(8), np.arange(8) + 3] = 1.0
-    out = R.head_shuffle(h, w, np.zeros(8), np.zeros((1, 4, 6, 8)))
-    assert np.array_equal(out, R.shuffle2(h[..., 3:11]))
+    x = rng.standard_normal((2, 2 * f, 3 * f, 4 * f))
+    k = f ** 3
+    y = R.unshuffle(x, f, k + 8)
+    assert y.shape == (2, 2, 3, 4, k + 8) and not y[..., k:].any()
+    for n, z, yy, xx, iz, iy, ix in np.ndindex(2, 2, 3, 4, f, f, f):
+        assert y[n, z, yy, xx, (iz * f + iy) * f + ix] == x[n, f * z + iz, f * yy + iy, f * xx + ix]
+    assert np.array_equal(R.shuffle(y, f), x)
+    # a head whose weights pick channel j + 3 of h puts it on block entry j
+    h = rng.standard_normal((1, 2, 3, 4, 80))
+    w = np.zeros((k, 80))
+    w[np.arange(k), np.arange(k) + 3] = 1.0
+    out = R.head_shuffle(h, w, np.zeros(k), np.zeros((1, 2 * f, 3 * f, 4 * f)), f)
+    assert np.array_equal(out, R.shuffle(h[..., 3:3 + k], f))
 
 
-def test_space_to_depth_head_adjoint_matches_autograd():
+@pytest.mark.parametrize("f", [2, 4])
+def test_space_to_depth_head_adjoint_matches_autograd(f):
     rng = np.random.default_rng(6)
+    k = f ** 3
     h = rng.standard_normal((2, 2, 3, 4, 32))
-    w = rng.standard_normal((8, 32))
-    b = rng.standard_normal(8)
-    x = rng.standard_normal((2, 4, 6, 8))
-    dout = rng.standard_normal((2, 4, 6, 8))
+    w = rng.standard_normal((k, 32))
+    b = rng.standard_normal(k)
+    x = rng.standard_normal((2, 2 * f, 3 * f, 4 * f))
+    dout = rng.standard_normal(x.shape)
     th, tw, tb = (torch.as_tensor(a).requires_grad_(True) for a in (h, w, b))
-    # depth-to-space through torch's own 2-D pixel_shuffle, applied per plane pair
-    s = (th @ tw.T + tb)                                     # n d h w 8
-    s = s.permute(0, 1, 4, 2, 3)                             # n d 8 h w, channel 4 iz + (2 iy + ix)
-    s = s.reshape(2, 2, 2, 4, 3, 4)                          # n d iz (iy ix) h w
-    planes = F.pixel_shuffle(s.reshape(2 * 2 * 2, 4, 3, 4), 2)  # (n d iz) 1 2h 2w
-    out = torch.as_tensor(x) + planes.reshape(2, 2, 2, 6, 8).reshape(2, 4, 6, 8)
-    assert np.abs(out.detach().numpy() - R.head_shuffle(h, w, b, x)).max() < 1e-12
+    # depth-to-space through torch's own 2-D pixel_shuffle, one fine plane iz at a time
+    s = (th @ tw.T + tb).permute(0, 1, 4, 2, 3)                 # n d k h w, k = iz f^2 + (iy f + ix)
+    s = s.reshape(2 * 2 * f, f * f, 3, 4)                       # (n d iz) (iy ix) h w
+    planes = F.pixel_shuffle(s, f)                              # (n d iz) 1 (f h) (f w)
+    out = torch.as_tensor(x) + planes.reshape(2, 2 * f, 3 * f, 4 * f)
+    assert np.abs(out.detach().numpy() - R.head_shuffle(h, w, b, x, f)).max() < 1e-12
     (out * torch.as_tensor(dout)).sum().backward()
-    dh, dw, db = R.head_shuffle_adjoint(dout, h, w)
+    dh, dw, db = R.head_shuffle_adjoint(dout, h, w, f)
     assert np.abs(dh - th.grad.numpy()).max() < 1e-12
     assert np.abs(dw - tw.grad.numpy()).max() < 1e-10
     assert np.abs(db - tb.grad.numpy()).max() < 1e-12
