@@ -209,32 +209,51 @@ __device__ __forceinline__ int sk_slot(int i) {
   return PAD ? i + (i >> 4) : i;
 }
 
-// One Stockham pass over N points in buf with BPT butterflies per thread;
-// H (optional) multiplies the values as they are read.
-template <int R, int BPT, bool PAD>
-__device__ __forceinline__ void sk_pass(float2* buf, int N, int Ns, float sign,
-                                        const float2* __restrict__ tw,
+// Buffer-resource word 3 for raw (untyped) loads on gfx9 (the twiddle table).
+constexpr int kTwRsrcWord3 = 0x00020000;
+
+// Slot distance of a run of d points, d a multiple of 16: the pad term of
+// sk_slot(i + d) then carries no bit from i, so sk_slot(i + d) = sk_slot(i) + sk_span(d).
+template <bool PAD>
+constexpr int sk_span(int d) {
+  return PAD ? d + d / 16 : d;
+}
+
+// One Stockham pass over N points in buf with BPT butterflies per thread
+// (stride NS between a butterfly's outputs); H (optional) multiplies the values
+// as they are read.  N, NS and the radix are compile-time, so every LDS access
+// is one per-thread base plus an immediate offset.
+template <int R, int BPT, bool PAD, int N, int NS>
+__device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* __restrict__ tw,
                                         const float* __restrict__ H) {
-  const int NB = N / R;
+  constexpr int NB = N / R;
+  static_assert(NB % 16 == 0, "a read stride keeps the pad term separable");
+  constexpr int kReadSpan = sk_span<PAD>(NB);
   float2 v[BPT][R];
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
     const int j = threadIdx.x + q * 256;
-    if (j < NB) {
+    if (NB >= 256 * (q + 1) || j < NB) {
+      const float2* src = buf + sk_slot<PAD>(j);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        float2 x = buf[sk_slot<PAD>(j + r * NB)];
+        float2 x = src[r * kReadSpan];
         if (H) {
           const float h = H[j + r * NB];
           x = make_float2(x.x * h, x.y * h);
         }
         v[q][r] = x;
       }
-      if (Ns > 1) {
-        const int step = (j & (Ns - 1)) * (N / (Ns * R));
+      if constexpr (NS > 1) {
+        // twiddle r of this butterfly: tw[step r], e^{-2 pi i step r / N}
+        const unsigned step8 = (unsigned)((j & (NS - 1)) * (N / (NS * R))) * 8u;
+        const __amdgpu_buffer_rsrc_t twr =
+            __builtin_amdgcn_make_buffer_rsrc((void*)tw, 0, N * 8, kTwRsrcWord3);
+        unsigned off = 0;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[step * r];  // e^{-2 pi i step r / N}
+          off += step8;
+          float2 w = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(twr, off, 0, 0));
           w.y *= -sign;
           v[q][r] = cmul(v[q][r], w);
         }
@@ -246,11 +265,22 @@ __device__ __forceinline__ void sk_pass(float2* buf, int N, int Ns, float sign,
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
     const int j = threadIdx.x + q * 256;
-    if (j < NB) {
-      const int k = j & (Ns - 1);
+    if (NB >= 256 * (q + 1) || j < NB) {
+      const int k = j & (NS - 1);
       const int base = (j - k) * R + k;
+      if constexpr (NS == 1) {
+        // base = j R: its R outputs stay inside one 16-point run (R <= 16, base % 16 + R <= 16)
+        float2* dst = buf + sk_slot<PAD>(base);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[sk_slot<PAD>(base + r * Ns)] = v[q][r];
+        for (int r = 0; r < R; ++r) dst[r] = v[q][r];
+      } else if constexpr (NS % 16 == 0) {
+        float2* dst = buf + sk_slot<PAD>(base);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * sk_span<PAD>(NS)] = v[q][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[sk_slot<PAD>(base + r * NS)] = v[q][r];
+      }
     }
   }
   __syncthreads();
@@ -268,27 +298,27 @@ __device__ __forceinline__ void fft_sk(float2* buf, const float2* tw, float sign
   constexpr int N = 1 << M;
   constexpr bool P = sk_pad<M>();
   if constexpr (M == 9) {
-    sk_pass<8, 1, P>(buf, N, 1, sign, tw, H);
-    sk_pass<8, 1, P>(buf, N, 8, sign, tw, nullptr);
-    sk_pass<8, 1, P>(buf, N, 64, sign, tw, nullptr);
+    sk_pass<8, 1, P, N, 1>(buf, sign, tw, H);
+    sk_pass<8, 1, P, N, 8>(buf, sign, tw, nullptr);
+    sk_pass<8, 1, P, N, 64>(buf, sign, tw, nullptr);
   } else if constexpr (M == 10) {
-    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
-    sk_pass<8, 1, P>(buf, N, 16, sign, tw, nullptr);
-    sk_pass<8, 1, P>(buf, N, 128, sign, tw, nullptr);
+    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
+    sk_pass<8, 1, P, N, 16>(buf, sign, tw, nullptr);
+    sk_pass<8, 1, P, N, 128>(buf, sign, tw, nullptr);
   } else if constexpr (M == 11) {
-    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
-    sk_pass<16, 1, P>(buf, N, 16, sign, tw, nullptr);
-    sk_pass<8, 1, P>(buf, N, 256, sign, tw, nullptr);
+    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
+    sk_pass<16, 1, P, N, 16>(buf, sign, tw, nullptr);
+    sk_pass<8, 1, P, N, 256>(buf, sign, tw, nullptr);
   } else if constexpr (M == 12) {
-    sk_pass<16, 1, P>(buf, N, 1, sign, tw, H);
-    sk_pass<16, 1, P>(buf, N, 16, sign, tw, nullptr);
-    sk_pass<16, 1, P>(buf, N, 256, sign, tw, nullptr);
+    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
+    sk_pass<16, 1, P, N, 16>(buf, sign, tw, nullptr);
+    sk_pass<16, 1, P, N, 256>(buf, sign, tw, nullptr);
   } else {
     static_assert(M == 13, "plans cover 2^9 .. 2^13");
-    sk_pass<16, 2, P>(buf, N, 1, sign, tw, H);
-    sk_pass<16, 2, P>(buf, N, 16, sign, tw, nullptr);
-    sk_pass<16, 2, P>(buf, N, 256, sign, tw, nullptr);
-    sk_pass<2, 16, P>(buf, N, 4096, sign, tw, nullptr);
+    sk_pass<16, 2, P, N, 1>(buf, sign, tw, H);
+    sk_pass<16, 2, P, N, 16>(buf, sign, tw, nullptr);
+    sk_pass<16, 2, P, N, 256>(buf, sign, tw, nullptr);
+    sk_pass<2, 16, P, N, 4096>(buf, sign, tw, nullptr);
   }
 }
 
@@ -345,17 +375,31 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
   if constexpr (M != 0) {
     // Every load of the thread's E points issued before any is used (clamped
     // addresses, pads selected afterwards), so one memory latency covers them.
+    // Row and cos-table loads go through buffer resources (one 32-bit offset
+    // per point serves all four); a missing cos table is a zero-size resource
+    // whose loads return 0 and are replaced by 1.
     constexpr int E = (1 << M) / 256;
     const float* sbb = has_b ? sb : sa;
     const float* cbb = has_b ? cb : ca;
+    const int row_bytes = nu * 4;
+    const __amdgpu_buffer_rsrc_t rxa = __builtin_amdgcn_make_buffer_rsrc((void*)sa, 0, row_bytes, kTwRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rxb = __builtin_amdgcn_make_buffer_rsrc((void*)sbb, 0, row_bytes, kTwRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rwa =
+        __builtin_amdgcn_make_buffer_rsrc((void*)ca, 0, ca ? row_bytes : 0, kTwRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rwb =
+        __builtin_amdgcn_make_buffer_rsrc((void*)cbb, 0, cbb ? row_bytes : 0, kTwRsrcWord3);
     float xa[E], xb[E], wa[E], wb[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const int p = min(max(threadIdx.x + 256 * i - off, 0), nu - 1);
-      xa[i] = sa[p];
-      xb[i] = sbb[p];
-      wa[i] = ca ? ca[p] : 1.f;
-      wb[i] = cbb ? cbb[p] : 1.f;
+      const unsigned po = (unsigned)min(max((int)threadIdx.x + 256 * i - off, 0), nu - 1) * 4u;
+      xa[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rxa, po, 0, 0));
+      xb[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rxb, po, 0, 0));
+      wa[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rwa, po, 0, 0));
+      wb[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rwb, po, 0, 0));
+    }
+    if (!ca) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) wa[i] = wb[i] = 1.f;
     }
 #pragma unroll
     for (int i = 0; i < E; ++i) {
