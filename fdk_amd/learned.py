@@ -147,6 +147,18 @@ def make_pairs(scanner: Scanner, seeds, region=256, sparse_step: int = 4, rng=No
     return xs, ys
 
 
+def lr_at(progress: float, lr: float, lr_final: float = 0.05, warmup: float = 0.0) -> float:
+    """Learning rate at `progress` in [0, 1] of a run: a cosine from lr down to
+    lr * lr_final, scaled by a linear ramp from lr_final to 1 over the first
+    `warmup` of the run."""
+    import math
+    p = min(max(progress, 0.0), 1.0)
+    r = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * p)))
+    if p < warmup:
+        r *= lr_final + (1.0 - lr_final) * p / warmup
+    return r
+
+
 def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
           log=None, max_seconds: float | None = None, lr_final: float = 0.05,
           warmup: float = 0.0):
@@ -155,7 +167,6 @@ def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, see
     (its progress is the larger of the step and time fractions), after a linear
     ramp from lr * lr_final over the first `warmup` of the run.  Returns the
     per-step mean losses."""
-    import math
     import torch
     rng = np.random.default_rng(seed)
     losses = []
@@ -171,9 +182,7 @@ def train(net: UNet3D, xs, ys, steps: int, batch: int = 4, lr: float = 1e-3, see
                     log(f"time budget reached after {s} steps")
                 break
         progress = min(max(s / max(steps - 1, 1), progress_t), 1.0)
-        lr_s = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * progress)))
-        if progress < warmup:
-            lr_s *= lr_final + (1.0 - lr_final) * progress / warmup
+        lr_s = lr_at(progress, lr, lr_final, warmup)
         idx = torch.as_tensor(rng.choice(xs.shape[0], size=batch, replace=False)).cuda()
         x = xs.index_select(0, idx)
         y = ys.index_select(0, idx)

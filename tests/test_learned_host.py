@@ -51,3 +51,23 @@ def test_weight_files_declare_their_architecture():
                 assert np.all(np.isfinite(z[f"conv{i}.w"]))
         seen += 1
     assert seen >= 2
+
+
+def test_learning_rate_schedule():
+    lr = 1e-3
+    assert Lr.lr_at(0.0, lr) == pytest.approx(lr)
+    assert Lr.lr_at(1.0, lr) == pytest.approx(0.05 * lr)
+    assert Lr.lr_at(0.5, lr) == pytest.approx(lr * (0.05 + 0.95 * 0.5))
+    # warm-up: starts at lr_final x the cosine value, ramps linearly, then joins the cosine
+    assert Lr.lr_at(0.0, lr, warmup=0.1) == pytest.approx(0.05 * lr)
+    assert Lr.lr_at(0.05, lr, warmup=0.1) == pytest.approx(Lr.lr_at(0.05, lr) * (0.05 + 0.95 * 0.5))
+    assert Lr.lr_at(0.1, lr, warmup=0.1) == pytest.approx(Lr.lr_at(0.1, lr))
+    vals = [Lr.lr_at(p / 100, lr, warmup=0.1) for p in range(101)]
+    assert all(b >= a for a, b in zip(vals[:10], vals[1:11]))      # rising through the warm-up
+    assert all(b <= a for a, b in zip(vals[10:-1], vals[11:]))     # then falling
+
+
+def test_patch_shapes():
+    assert Lr._dhw(128) == (128, 128, 128)
+    assert Lr._dhw((256, 256, 512)) == (256, 256, 512)
+    assert Lr._dhw([64, 32, 16]) == (64, 32, 16)
