@@ -97,38 +97,60 @@ __device__ inline float tap_sample(const float* __restrict__ vol, int nx, int ny
 // arithmetic as tap_sample.
 constexpr int kFpRsrcWord3 = 0x00020000;
 
+// The paired-load trilinear sample in two halves, so that several samples'
+// loads can be in flight before any is combined: tap_fetch_buf issues the four
+// 8-byte loads (pair element xb, xb + 1 of each row) and keeps what the blend
+// needs; tap_blend forms the sample.
+struct TapBuf {
+  float2 q[4];              // rows (z0,y0), (z0,y1), (z1,y0), (z1,y1)
+  float fx, fy, fz;
+  float mx0, mx1, my0, my1, mz0, mz1;
+  bool s0, s1;              // tap x0 / x1 is the pair's second element
+};
+
 template <int MODE>
-__device__ inline float tap_sample_buf(__amdgpu_buffer_rsrc_t rv, int nx, int ny, int nz, float px,
+__device__ inline TapBuf tap_fetch_buf(__amdgpu_buffer_rsrc_t rv, int nx, int ny, int nz, float px,
                                        float py, float pz) {
+  TapBuf b;
   const int ax = (int)px, ay = (int)py, az = (int)pz;  // truncation, as the reference
-  const float fx = px - ax, fy = py - ay, fz = pz - az;
+  b.fx = px - ax;
+  b.fy = py - ay;
+  b.fz = pz - az;
   int y0 = ay, y1 = ay + 1, z0 = az, z1 = az + 1;
-  float mx0 = 1.f, mx1 = 1.f, my0 = 1.f, my1 = 1.f, mz0 = 1.f, mz1 = 1.f;
+  b.mx0 = b.mx1 = b.my0 = b.my1 = b.mz0 = b.mz1 = 1.f;
   if (MODE != FDK_FP_CLAMP) {
-    mx0 = (ax >= 0 && ax < nx) ? 1.f : 0.f; mx1 = (ax + 1 >= 0 && ax + 1 < nx) ? 1.f : 0.f;
-    my0 = (y0 >= 0 && y0 < ny) ? 1.f : 0.f; my1 = (y1 >= 0 && y1 < ny) ? 1.f : 0.f;
-    mz0 = (z0 >= 0 && z0 < nz) ? 1.f : 0.f; mz1 = (z1 >= 0 && z1 < nz) ? 1.f : 0.f;
+    b.mx0 = (ax >= 0 && ax < nx) ? 1.f : 0.f; b.mx1 = (ax + 1 >= 0 && ax + 1 < nx) ? 1.f : 0.f;
+    b.my0 = (y0 >= 0 && y0 < ny) ? 1.f : 0.f; b.my1 = (y1 >= 0 && y1 < ny) ? 1.f : 0.f;
+    b.mz0 = (z0 >= 0 && z0 < nz) ? 1.f : 0.f; b.mz1 = (z1 >= 0 && z1 < nz) ? 1.f : 0.f;
   }
   y0 = min(max(y0, 0), ny - 1); y1 = min(max(y1, 0), ny - 1);
   z0 = min(max(z0, 0), nz - 1); z1 = min(max(z1, 0), nz - 1);
   const int x0 = min(max(ax, 0), nx - 1), x1 = min(max(ax + 1, 0), nx - 1);
   const int xb = min(max(ax, 0), nx - 2);
-  const bool s0 = x0 != xb, s1 = x1 != xb;  // the tap is the pair's second element
+  b.s0 = x0 != xb;
+  b.s1 = x1 != xb;
   const int pz0 = z0 * ny * nx, pz1 = z1 * ny * nx;
-  const int o00 = (pz0 + y0 * nx + xb) * 4, o01 = (pz0 + y1 * nx + xb) * 4;
-  const int o10 = (pz1 + y0 * nx + xb) * 4, o11 = (pz1 + y1 * nx + xb) * 4;
-  const float2 q00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o00, 0, 0));
-  const float2 q01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o01, 0, 0));
-  const float2 q10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o10, 0, 0));
-  const float2 q11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o11, 0, 0));
-  const float t000 = (s0 ? q00.y : q00.x) * (mx0 * my0 * mz0), t001 = (s1 ? q00.y : q00.x) * (mx1 * my0 * mz0);
-  const float t010 = (s0 ? q01.y : q01.x) * (mx0 * my1 * mz0), t011 = (s1 ? q01.y : q01.x) * (mx1 * my1 * mz0);
-  const float t100 = (s0 ? q10.y : q10.x) * (mx0 * my0 * mz1), t101 = (s1 ? q10.y : q10.x) * (mx1 * my0 * mz1);
-  const float t110 = (s0 ? q11.y : q11.x) * (mx0 * my1 * mz1), t111 = (s1 ? q11.y : q11.x) * (mx1 * my1 * mz1);
-  const float gx = 1.f - fx, gy = 1.f - fy;
-  const float v0 = (t000 * gx + t001 * fx) * gy + (t010 * gx + t011 * fx) * fy;
-  const float v1 = (t100 * gx + t101 * fx) * gy + (t110 * gx + t111 * fx) * fy;
-  return v0 * (1.f - fz) + v1 * fz;
+  const int o[4] = {(pz0 + y0 * nx + xb) * 4, (pz0 + y1 * nx + xb) * 4, (pz1 + y0 * nx + xb) * 4,
+                    (pz1 + y1 * nx + xb) * 4};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    b.q[k] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rv, o[k], 0, 0));
+  return b;
+}
+
+__device__ inline float tap_blend(const TapBuf& b) {
+  const float t000 = (b.s0 ? b.q[0].y : b.q[0].x) * (b.mx0 * b.my0 * b.mz0);
+  const float t001 = (b.s1 ? b.q[0].y : b.q[0].x) * (b.mx1 * b.my0 * b.mz0);
+  const float t010 = (b.s0 ? b.q[1].y : b.q[1].x) * (b.mx0 * b.my1 * b.mz0);
+  const float t011 = (b.s1 ? b.q[1].y : b.q[1].x) * (b.mx1 * b.my1 * b.mz0);
+  const float t100 = (b.s0 ? b.q[2].y : b.q[2].x) * (b.mx0 * b.my0 * b.mz1);
+  const float t101 = (b.s1 ? b.q[2].y : b.q[2].x) * (b.mx1 * b.my0 * b.mz1);
+  const float t110 = (b.s0 ? b.q[3].y : b.q[3].x) * (b.mx0 * b.my1 * b.mz1);
+  const float t111 = (b.s1 ? b.q[3].y : b.q[3].x) * (b.mx1 * b.my1 * b.mz1);
+  const float gx = 1.f - b.fx, gy = 1.f - b.fy;
+  const float v0 = (t000 * gx + t001 * b.fx) * gy + (t010 * gx + t011 * b.fx) * b.fy;
+  const float v1 = (t100 * gx + t101 * b.fx) * gy + (t110 * gx + t111 * b.fx) * b.fy;
+  return v0 * (1.f - b.fz) + v1 * b.fz;
 }
 
 template <int MODE, bool BUF>
@@ -155,16 +177,41 @@ __global__ __launch_bounds__(256) void fp_kernel(RayGeom g, const float* __restr
                 oz = (float)(g.cz - o[2] / g.vz);
     const float sx = (float)(d[0] / g.vx), sy = (float)(-d[1] / g.vy), sz = (float)(-d[2] / g.vz);
     const float tfar = (float)tf;
-    if constexpr (BUF) {
-      const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)vol, 0, g.nx * g.ny * g.nz * 4, kFpRsrcWord3);
-      for (float t = (float)tn; MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar; t += step)
-        sum += tap_sample_buf<MODE>(rv, g.nx, g.ny, g.nz, fmaf(sx, t, ox), fmaf(sy, t, oy),
-                                    fmaf(sz, t, oz));
-    } else {
-      for (float t = (float)tn; MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar; t += step)
-        sum += tap_sample<MODE>(vol, g.nx, g.ny, g.nz, fmaf(sx, t, ox), fmaf(sy, t, oy),
-                                fmaf(sz, t, oz));
+    // Four samples per iteration: their gathers are issued together, so one
+    // memory latency covers four steps.  t advances by the same float additions
+    // and the samples are summed in the same order as a one-step loop, and a
+    // sample past tfar adds nothing, so the result is the one-step loop's.
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)vol, 0, BUF ? g.nx * g.ny * g.nz * 4 : 0, kFpRsrcWord3);
+    float t = (float)tn;
+    bool more = MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar;
+    while (more) {
+      float tk[4], sk[4];
+      bool ok[4];
+      tk[0] = t;
+#pragma unroll
+      for (int k = 1; k < 4; ++k) tk[k] = tk[k - 1] + step;
+      if constexpr (BUF) {
+        TapBuf tb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          tb[k] = tap_fetch_buf<MODE>(rv, g.nx, g.ny, g.nz, fmaf(sx, tk[k], ox),
+                                      fmaf(sy, tk[k], oy), fmaf(sz, tk[k], oz));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sk[k] = tap_blend(tb[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          sk[k] = tap_sample<MODE>(vol, g.nx, g.ny, g.nz, fmaf(sx, tk[k], ox),
+                                   fmaf(sy, tk[k], oy), fmaf(sz, tk[k], oz));
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ok[k] = MODE == FDK_FP_CLAMP ? tk[k] < tfar : tk[k] <= tfar;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (ok[k]) sum += sk[k];
+      t = tk[3] + step;
+      more = ok[3] && (MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar);
     }
   }
   out[((long long)sv * g.nv + j) * g.nu + i] = sum * step;
