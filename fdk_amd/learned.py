@@ -2,7 +2,7 @@
 counterpart).  The reference's sparse-view mechanism is loading every step-th
 view of a scan (cuFDK/cuFDK.cpp:62-94, common/type.h:4847-4851); here FDK from
 every 4th view is followed by a 3-D residual U-Net trained to map it to the
-full-view FDK.  Plain full-view FDK stays both the fallback and the judge: every
+full-view FDK.  Plain full-view FDK stays both the fallback and the yardstick: every
 comparison is against it.
 
 Pipeline pieces:
