@@ -368,20 +368,21 @@ __device__ __forceinline__ Fwd4Tile fwd4_tile(const Conv& a, int tile, int tiles
 
 // Loads the halo of (tile, 32-channel chunk c0) into registers: slot s = t + 256 j
 // is halo row s / 136 (plane-major), x s % 136 / 4 and 8-channel group s % 4.
-// Interior tiles (uniform) skip the range checks.
+// Offsets are signed and relative to the tile's first voxel (z0, y0, x0), which
+// is always inside the volume; slots outside it are never addressed.  Interior
+// tiles (uniform) skip the range checks.
 __device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a, const Fwd4Tile& f,
                                            int c0) {
   const int t = threadIdx.x;
-  const u16* org = a.x + ((((long long)f.n * a.d + f.z0 - 1) * a.h + f.y0 - 1) * a.w + f.x0 - 1) *
-                             a.cin + c0;
-  const unsigned plane = (unsigned)a.h * (unsigned)a.w * (unsigned)a.cin;
-  const unsigned rowe = (unsigned)a.w * (unsigned)a.cin;
+  const u16* org =
+      a.x + ((((long long)f.n * a.d + f.z0) * a.h + f.y0) * a.w + f.x0) * a.cin + c0;
+  const int plane = a.h * a.w * a.cin, rowe = a.w * a.cin;
   int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
   if (f.interior) {
 #pragma unroll
     for (int j = 0; j < kHaloPer; ++j) {
       const int hz = row / kHaloRows, hy = row - hz * kHaloRows;
-      const unsigned off = hz * plane + hy * rowe + (unsigned)(w >> 2) * a.cin + 8 * (w & 3);
+      const int off = (hz - 1) * plane + (hy - 1) * rowe + ((w >> 2) - 1) * a.cin + 8 * (w & 3);
       if (j + 1 < kHaloPer || t + 256 * j < kHaloChunks)
         pre[j] = *reinterpret_cast<const uint4*>(org + off);
       w += 256 - kHaloRowSlots;  // 256 = one row (136 slots) + 120
@@ -399,7 +400,7 @@ __device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a
     const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
     const bool ok = (j + 1 < kHaloPer || t + 256 * j < kHaloChunks) && zz >= 0 && zz < a.d &&
                     yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
-    const unsigned off = hz * plane + hy * rowe + (unsigned)hx * a.cin + 8 * (w & 3);
+    const int off = (hz - 1) * plane + (hy - 1) * rowe + (hx - 1) * a.cin + 8 * (w & 3);
     pre[j] = ok ? *reinterpret_cast<const uint4*>(org + off) : make_uint4(0u, 0u, 0u, 0u);
     w += 256 - kHaloRowSlots;
     row += 1;
