@@ -328,7 +328,10 @@ def test_unet_training_reduces_loss(torch_cuda, s2d, act):
         # a smooth corruption instead
         noisy = clean + (0.6 * np.cos(4 * x) * np.sin(3 * z) + 0.2)[None].astype(np.float32)
     xd, td = torch.as_tensor(noisy).cuda(), torch.as_tensor(clean).cuda()
-    steps = 120 if s2d > 1 else 40  # the coarse-grid nets start further from a smoother
+    # the coarse-grid nets start further from a smoother; single steps are noisy
+    # (atomic weight-gradient sums), so the last ten are averaged
+    steps = {1: 40, 2: 240, 4: 120}[s2d]
     losses = [net.train_step(xd, td, lr=1e-3).item() for _ in range(steps)]
-    print("training loss:", [round(l, 1) for l in losses[::steps // 5]], "->", round(losses[-1], 1))
-    assert losses[-1] < 0.5 * losses[0]
+    tail = float(np.mean(losses[-10:]))
+    print("training loss:", [round(l, 1) for l in losses[::steps // 5]], "-> last ten", round(tail, 1))
+    assert tail < 0.5 * losses[0]
