@@ -125,10 +125,6 @@ __device__ inline float sample_v(const void* rows, float v, int W, float w0, flo
   return ok ? s : 0.0f;
 }
 
-// Buffer-resource word 3 for untyped (raw) dword access on gfx9 (CDNA): data
-// format 32, no swizzle.
-constexpr int kRsrcWord3 = 0x00020000;
-
 // FP32 sample through the row-u / row-u+1 resources: one 32-bit offset, two
 // 8-byte loads (taps v, v+1 of each row).  0 outside 0 <= v < W-1.
 __device__ inline float sample_buf(__amdgpu_buffer_rsrc_t r0, __amdgpu_buffer_rsrc_t r1, float v,
@@ -245,9 +241,9 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
       // reads 0 instead of touching memory.  Lane offsets are 32-bit.
       const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
       const __amdgpu_buffer_rsrc_t r0 =
-          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, fdk::kRawRsrcWord3);
       const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, fdk::kRawRsrcWord3);
       const int rowbase = uok ? iu * a.W : 0;
 #pragma unroll
       for (int j = 0; j < ZPT; ++j) {
@@ -257,9 +253,9 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
     } else if (a.buf16) {
       const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
       const __amdgpu_buffer_rsrc_t r0 =
-          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+          __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, fdk::kRawRsrcWord3);
       const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+          (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, fdk::kRawRsrcWord3);
       const int rowbase = uok ? iu * a.W : 0;
 #pragma unroll
       for (int j = 0; j < ZPT; ++j) {
@@ -565,9 +561,9 @@ __device__ __forceinline__ void compute_direct(float (&acc)[ZPT], float (&accm)[
                                                const ColView& c, const BpArgs& a,
                                                const char* vbase, float sc, float of) {
   const __amdgpu_buffer_rsrc_t g0 =
-      __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+      __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, fdk::kRawRsrcWord3);
   const __amdgpu_buffer_rsrc_t g1 = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, kRsrcWord3);
+      (void*)(vbase + a.row_bytes), 0, a.view_bytes32 - a.row_bytes, fdk::kRawRsrcWord3);
   const int rowbase = c.iu * a.W;
   const float off = of * c.Z2;
 #pragma unroll
@@ -641,7 +637,7 @@ __device__ __forceinline__ void column_write(const BpArgs& a, int xs, int ys, in
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const BpArgs& a, int s) {
   const char* vbase = static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes;
-  return __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, kRsrcWord3);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)vbase, 0, a.view_bytes32, fdk::kRawRsrcWord3);
 }
 
 // Stage, then compute, every view (two block barriers per view).

@@ -13,6 +13,10 @@
 
 namespace fdk {
 
+// Word 3 of a buffer resource for raw (untyped) dword access on gfx9 / CDNA:
+// data format 32, no swizzle.  Loads past the resource's byte size return 0.
+constexpr int kRawRsrcWord3 = 0x00020000;
+
 // Records a printf-style message as the calling thread's last error and
 // returns code, so call sites can `return fail(...)`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));

@@ -209,9 +209,6 @@ __device__ __forceinline__ int sk_slot(int i) {
   return PAD ? i + (i >> 4) : i;
 }
 
-// Buffer-resource word 3 for raw (untyped) loads on gfx9 (the twiddle table).
-constexpr int kTwRsrcWord3 = 0x00020000;
-
 // Slot distance of a run of d points, d a multiple of 16: the pad term of
 // sk_slot(i + d) then carries no bit from i, so sk_slot(i + d) = sk_slot(i) + sk_span(d).
 template <bool PAD>
@@ -248,7 +245,7 @@ __device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* _
         // twiddle r of this butterfly: tw[step r], e^{-2 pi i step r / N}
         const unsigned step8 = (unsigned)((j & (NS - 1)) * (N / (NS * R))) * 8u;
         const __amdgpu_buffer_rsrc_t twr =
-            __builtin_amdgcn_make_buffer_rsrc((void*)tw, 0, N * 8, kTwRsrcWord3);
+            __builtin_amdgcn_make_buffer_rsrc((void*)tw, 0, N * 8, fdk::kRawRsrcWord3);
         unsigned off = 0;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -382,12 +379,12 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
     const float* sbb = has_b ? sb : sa;
     const float* cbb = has_b ? cb : ca;
     const int row_bytes = nu * 4;
-    const __amdgpu_buffer_rsrc_t rxa = __builtin_amdgcn_make_buffer_rsrc((void*)sa, 0, row_bytes, kTwRsrcWord3);
-    const __amdgpu_buffer_rsrc_t rxb = __builtin_amdgcn_make_buffer_rsrc((void*)sbb, 0, row_bytes, kTwRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rxa = __builtin_amdgcn_make_buffer_rsrc((void*)sa, 0, row_bytes, fdk::kRawRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rxb = __builtin_amdgcn_make_buffer_rsrc((void*)sbb, 0, row_bytes, fdk::kRawRsrcWord3);
     const __amdgpu_buffer_rsrc_t rwa =
-        __builtin_amdgcn_make_buffer_rsrc((void*)ca, 0, ca ? row_bytes : 0, kTwRsrcWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void*)ca, 0, ca ? row_bytes : 0, fdk::kRawRsrcWord3);
     const __amdgpu_buffer_rsrc_t rwb =
-        __builtin_amdgcn_make_buffer_rsrc((void*)cbb, 0, cbb ? row_bytes : 0, kTwRsrcWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void*)cbb, 0, cbb ? row_bytes : 0, fdk::kRawRsrcWord3);
     float xa[E], xb[E], wa[E], wb[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {

@@ -94,13 +94,9 @@ __device__ inline float tap_sample(const float* __restrict__ vol, int nx, int ny
 // volumes under 2 GiB): the taps x0 and x1 of each (y, z) row come from one
 // 8-byte load at xb = clamp(ax, 0, nx - 2), each tap picking the element it
 // clamps to, so a sample costs 4 loads instead of 8.  Same values and
-// arithmetic as tap_sample.
-constexpr int kFpRsrcWord3 = 0x00020000;
-
-// The paired-load trilinear sample in two halves, so that several samples'
+// arithmetic as tap_sample.  It comes in two halves, so that several samples'
 // loads can be in flight before any is combined: tap_fetch_buf issues the four
-// 8-byte loads (pair element xb, xb + 1 of each row) and keeps what the blend
-// needs; tap_blend forms the sample.
+// loads and keeps what the blend needs; tap_blend forms the sample.
 struct TapBuf {
   float2 q[4];              // rows (z0,y0), (z0,y1), (z1,y0), (z1,y1)
   float fx, fy, fz;
@@ -182,7 +178,7 @@ __global__ __launch_bounds__(256) void fp_kernel(RayGeom g, const float* __restr
     // and the samples are summed in the same order as a one-step loop, and a
     // sample past tfar adds nothing, so the result is the one-step loop's.
     const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)vol, 0, BUF ? g.nx * g.ny * g.nz * 4 : 0, kFpRsrcWord3);
+        (void*)vol, 0, BUF ? g.nx * g.ny * g.nz * 4 : 0, fdk::kRawRsrcWord3);
     float t = (float)tn;
     bool more = MODE == FDK_FP_CLAMP ? t < tfar : t <= tfar;
     while (more) {

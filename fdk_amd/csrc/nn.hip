@@ -333,7 +333,6 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
 //    loaded into registers while the current one's taps run.
 // Staging offsets are 32-bit (4 planes of the input fit in 2^31 elements).
 constexpr int kHaloPer = (kHaloChunks + 255) / 256;  // halo slots per thread
-constexpr int kWeightRsrcWord3 = 0x00020000;  // raw (untyped) buffer access, gfx9 format word
 constexpr int kHaloRowSlots = 4 * kHaloX;            // 16-byte slots per halo row
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -464,7 +463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const int pz = wave >> 1, py = 2 * (wave & 1);  // the wave's plane and first row in the tile
   const uint4* hb = halo + ((pz * kHaloRows + py) * 4 + kq) * kHaloX + m;
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), kWeightRsrcWord3);
+      (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), fdk::kRawRsrcWord3);
   const unsigned lane_off = (unsigned)(kq * 8 + m * a.cin) * 2u;
   const unsigned tap_bytes = (unsigned)(a.cout * a.cin) * 2u, nt_bytes = (unsigned)(16 * a.cin) * 2u;
   const int nchunks = a.cin / 32;
