@@ -318,8 +318,8 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
       }
 }
 
-// Lean form of the LDS-halo kernel, used for cout = 32 (NT = 2; at NT = 4 its
-// registers spill), on the same
+// Lean form of the LDS-halo kernel for cout = 32 (NT = 2, PF) and 64 (NT = 4,
+// no PF: its registers cannot also hold the next halo), on the same
 // tiles (2 planes x 4 rows x 32 voxels per block, 1 plane x 2 rows per wave)
 // and halo layout as conv3d_fwd3_kernel:
 //  * the 27 taps are unrolled, so every A-fragment read is one ds_read_b128
@@ -329,8 +329,9 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
 //  * the MFMA takes the weights as its A operand and the voxels as B, so lane
 //    (m, kq) ends with output channels 4 kq .. 4 kq + 3 of voxel m: one 8-byte
 //    BF16 store (16-byte FP32) per lane and tile instead of four 2-byte ones;
-//  * a block walks `tpb` consecutive tiles; the next (tile, chunk) halo is
-//    loaded into registers while the current one's taps run.
+//  * a block walks `tpb` consecutive tiles; with PF the next (tile, chunk) halo
+//    is loaded into registers while the current one's taps run, without it
+//    each halo is loaded right before it is staged.
 // Staging offsets are 32-bit (4 planes of the input fit in 2^31 elements).
 constexpr int kHaloPer = (kHaloChunks + 255) / 256;  // halo slots per thread
 constexpr int kHaloRowSlots = 4 * kHaloX;            // 16-byte slots per halo row
@@ -452,7 +453,7 @@ __device__ __forceinline__ void fwd4_read_a(bf16x8 (&af)[4], const uint4* hb, in
         bf16x8, hb[(dz * kHaloRows + (mt >> 1) + dy) * kRowStride + 16 * (mt & 1) + dx]);
 }
 
-template <int NT>
+template <int NT, bool PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_fwd4_kernel(
     Conv a, int tiles_x, int tiles_y, int ntiles, int tpb) {
   constexpr int P = NT == 2 ? 3 : 2;  // weight prefetch distance in taps
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const int nchunks = a.cin / 32;
   const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
   uint4 pre[kHaloPer];
-  if (first < last) fwd4_fetch(pre, a, fwd4_tile(a, first, tiles_x, tiles_y), 0);
+  if (PF && first < last) fwd4_fetch(pre, a, fwd4_tile(a, first, tiles_x, tiles_y), 0);
   for (int tile = first; tile < last; ++tile) {
     const Fwd4Tile f = fwd4_tile(a, tile, tiles_x, tiles_y);
     f32x4 acc[4][NT];
@@ -478,6 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ch = 0; ch < nchunks; ++ch) {
+      if (!PF) fwd4_fetch(pre, a, f, ch * 32);  // staged synchronously: no registers held
       __syncthreads();  // every wave is done reading the previous halo
       fwd4_commit(halo, pre);
       __syncthreads();
@@ -486,7 +488,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
       for (int tp = 0; tp < P; ++tp) fwd4_load_w<NT>(b[tp], wr, lane_off, soff, nt_bytes, tap_bytes);
       // the next halo's loads run under this chunk's taps
-      {
+      if (PF) {
         int nt_ = tile, nc = ch + 1;
         if (nc == nchunks) {
           nc = 0;
@@ -1224,7 +1226,7 @@ extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void*
   return FDK_OK;
 }
 
-// Forward convolution kernel: the lean LDS-halo kernel (cout 32) or the
+// Forward convolution kernel: the lean LDS-halo kernel (cout 32 / 64) or the
 // LDS-halo kernel where their shape rules hold (default), else the pipelined
 // 2-row tile, else the 1-row kernel; FDK_NN_CONV=3, 2 or 1 caps the choice.
 static int conv_kernel_choice() {
@@ -1255,7 +1257,7 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   dim3 grid(wd / 32, h / 4, n * d);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ck = conv_kernel_choice();
-  const bool lean = ck >= 4 && cin % 32 == 0 && d % 2 == 0 && cout == 32 &&
+  const bool lean = ck >= 4 && cin % 32 == 0 && d % 2 == 0 && (cout == 32 || cout == 64) &&
                     aligned16(x) && aligned16(w) && aligned16(y) && (!bias || aligned16(bias)) &&
                     (!resid || aligned16(resid)) &&
                     ((4LL * h * wd + 64) * cin < (1LL << 31)) && 27LL * cout * cin * 2 < (1LL << 31);
@@ -1263,7 +1265,10 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
     const int tiles_x = wd / 32, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
     const int tpb = 2;
     const dim3 g4((ntiles + tpb - 1) / tpb);
-    hipLaunchKernelGGL(conv3d_fwd4_kernel<2>, g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+    if (cout == 32)
+      hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+    else
+      hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
     FDK_HIP(hipGetLastError());
     return FDK_OK;
   }
