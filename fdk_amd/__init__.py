@@ -98,7 +98,13 @@ _SIGS = {
                          + [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                             C.c_void_p]),
     "fdk_filter_proj_image": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
-                                        C.c_void_p, C.c_int, C.c_int, C.c_int]),
+                                        C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "fdk_dev_malloc": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "fdk_dev_free": (C.c_int, [C.c_void_p]),
+    "fdk_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "fdk_memcpy_2d": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                                C.c_size_t]),
+    "fdk_dev_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t]),
     "fdk_filter_frame_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "fdk_fp_project_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
@@ -159,6 +165,71 @@ def _np_ptr(a: np.ndarray):
 
 def _dev_ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+_F32_NAMES = ("torch.float32",)
+_U16_NAMES = ("torch.int16", "torch.uint16", "torch.float16", "torch.bfloat16")
+
+
+def _dev_buffer(t, what: str, nbytes: int, dtype: int | None = FDK_F32) -> int:
+    """Checks that ``t`` is a contiguous CUDA tensor holding at least nbytes
+    (and, when dtype is given, of an element type matching that stored dtype:
+    float32 for FDK_F32, a 16-bit type for FDK_F16 / FDK_BF16); returns its
+    device address.  A launch over a too-small tensor would read or write
+    outside its allocation, so every device wrapper checks before launching."""
+    if not hasattr(t, "data_ptr") or not hasattr(t, "is_cuda"):
+        raise FdkError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise FdkError(f"{what}: tensor is not on the GPU")
+    if not t.is_contiguous():
+        raise FdkError(f"{what}: tensor is not contiguous")
+    if dtype is not None:
+        names = _F32_NAMES if dtype == FDK_F32 else _U16_NAMES
+        if str(t.dtype) not in names:
+            raise FdkError(f"{what}: element type {t.dtype} does not match the stored dtype "
+                           f"({'float32' if dtype == FDK_F32 else '16-bit'})")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise FdkError(f"{what}: tensor holds {have} bytes, the call needs {nbytes}")
+    return t.data_ptr()
+
+
+def _host_array(a, what: str, nbytes: int, elem: int) -> C.c_void_p:
+    """Host buffer (numpy array or CPU tensor), contiguous, of element size
+    ``elem`` and at least nbytes; returns its address."""
+    if hasattr(a, "data_ptr"):
+        if a.is_cuda:
+            raise FdkError(f"{what}: expected host memory, got a GPU tensor")
+        if not a.is_contiguous():
+            raise FdkError(f"{what}: host array is not contiguous")
+        have, esz, ptr = a.numel() * a.element_size(), a.element_size(), a.data_ptr()
+    elif isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise FdkError(f"{what}: host array is not contiguous")
+        have, esz, ptr = a.nbytes, a.itemsize, a.ctypes.data
+    else:
+        raise FdkError(f"{what}: expected a numpy array or a CPU tensor")
+    if esz != elem:
+        raise FdkError(f"{what}: element size {esz} bytes, expected {elem}")
+    if have < nbytes:
+        raise FdkError(f"{what}: host array holds {have} bytes, the call needs {nbytes}")
+    return C.c_void_p(ptr)
+
+
+def _mats(mats, n: int, what: str) -> np.ndarray:
+    m = np.ascontiguousarray(mats, np.float32).reshape(-1)
+    if m.size < 12 * n:
+        raise FdkError(f"{what}: {m.size // 12} matrices for {n} views")
+    return m
+
+
+def _scale_offset(so, n: int, what: str):
+    if so is None:
+        return None
+    a = np.ascontiguousarray(so, np.float32).reshape(-1)
+    if a.size < 2 * n:
+        raise FdkError(f"{what}: {a.size // 2} scale/offset pairs for {n} views")
+    return a
 
 
 def _stream(stream=None):
@@ -295,8 +366,10 @@ class BackProjectionFDK:
     def BP(self, transpose: int, proj: np.ndarray, width: int, height: int, mats: np.ndarray,
            nProj: int) -> bool:
         proj = np.ascontiguousarray(proj, np.float32)
-        mats = np.ascontiguousarray(mats, np.float32)
-        assert proj.size >= width * height * nProj and mats.size >= 12 * nProj
+        mats = _mats(mats, nProj, "BackProjectionFDK::BP")
+        if proj.size < width * height * nProj:
+            raise FdkError(f"BackProjectionFDK::BP: {proj.size} samples for {nProj} views of "
+                           f"{width} x {height}")
         _check(lib().fdk_bp_batch(self._h, int(transpose), _np_ptr(proj), int(width), int(height),
                                   _np_ptr(mats), int(nProj)), "BackProjectionFDK::BP")
         return True
@@ -305,13 +378,19 @@ class BackProjectionFDK:
                   transpose: int = 1, dtype: int = FDK_F32, view_stride: int | None = None,
                   scale_offset: np.ndarray | None = None, offset_views: int = 0):
         """Backproject nProj device-resident views (a torch tensor) asynchronously."""
-        mats = np.ascontiguousarray(mats, np.float32)
+        if dtype not in (FDK_F32, FDK_F16, FDK_BF16):
+            raise FdkError(f"bp_device: unknown dtype {dtype}")
+        mats = _mats(mats, nProj, "bp_device")
         vs = width * height if view_stride is None else int(view_stride)
+        if vs < width * height or offset_views < 0:
+            raise FdkError("bp_device: view stride below a view, or negative view offset")
         esize = 4 if dtype == FDK_F32 else 2
-        ptr = C.c_void_p(proj.data_ptr() + offset_views * vs * esize)
-        so = None
-        if scale_offset is not None:
-            so = np.ascontiguousarray(scale_offset, np.float32)
+        need = ((offset_views + nProj - 1) * vs + width * height) * esize if nProj > 0 else 0
+        ptr = C.c_void_p(_dev_buffer(proj, "bp_device proj", need, dtype) +
+                         offset_views * vs * esize)
+        so = _scale_offset(scale_offset, nProj, "bp_device")
+        if dtype != FDK_F32 and so is None and nProj > 0:
+            raise FdkError("bp_device: 16-bit views need their scale_offset")
         _check(lib().fdk_bp_batch_device(self._h, int(transpose), ptr, int(dtype), int(width),
                                          int(height), vs, _np_ptr(mats),
                                          _np_ptr(so) if so is not None else None, int(nProj)),
@@ -353,6 +432,7 @@ class BackProjectionFDK:
         return t.view(*dims)
 
     def get_volume_device(self, out):
+        _dev_buffer(out, "get_volume_device", 0)
         _check(lib().fdk_bp_get_volume_device(self._h, _dev_ptr(out), out.numel()),
                "get_volume_device")
         return out
@@ -381,31 +461,22 @@ class Pipeline:
                                      int(slots), C.byref(h)), "Pipeline")
         self._h = h
         self.store_dtype = store_dtype
+        self._view = para.nu * para.nv
         self.bp = BackProjectionFDK._borrowed(lib().fdk_pipe_bp(h), para.nx, para.ny, para.nz,
                                               sub_top, sub_bottom, self)
 
-    @staticmethod
-    def _host_ptr(a):
-        if hasattr(a, "data_ptr"):
-            if a.is_cuda:
-                raise ValueError("Pipeline takes host arrays")
-            if not a.is_contiguous():
-                raise ValueError("host views must be contiguous")
-            return C.c_void_p(a.data_ptr())
-        if not a.flags["C_CONTIGUOUS"]:
-            raise ValueError("host views must be contiguous")
-        return C.c_void_p(a.ctypes.data)
-
     def push_raw(self, views, count: int, first_view: int, view_step: int = 1):
-        _check(lib().fdk_pipe_push_raw(self._h, self._host_ptr(views), int(count),
-                                       int(first_view), int(view_step)), "push_raw")
+        ptr = _host_array(views, "push_raw", int(count) * self._view * 4, 4)
+        _check(lib().fdk_pipe_push_raw(self._h, ptr, int(count), int(first_view), int(view_step)),
+               "push_raw")
 
     def push_filtered(self, views, count: int, first_view: int, view_step: int = 1,
                       scale_offset=None):
-        so = None if scale_offset is None else np.ascontiguousarray(scale_offset, np.float32)
-        _check(lib().fdk_pipe_push_filtered(self._h, self._host_ptr(views), int(count),
-                                            int(first_view), int(view_step),
-                                            _np_ptr(so) if so is not None else None),
+        esize = 4 if self.store_dtype == FDK_F32 else 2
+        ptr = _host_array(views, "push_filtered", int(count) * self._view * esize, esize)
+        so = _scale_offset(scale_offset, int(count), "push_filtered")
+        _check(lib().fdk_pipe_push_filtered(self._h, ptr, int(count), int(first_view),
+                                            int(view_step), _np_ptr(so) if so is not None else None),
                "push_filtered")
 
     def finish(self):
@@ -475,10 +546,18 @@ def reconstruct_conebeam(proj_size, vol_size, mats, vol_in, vol_out, proj, strea
     tensors, proj a device tensor or a host numpy array."""
     ps = (C.c_int * 3)(*proj_size)
     vs = (C.c_int * 3)(*vol_size)
-    mats = np.ascontiguousarray(mats, np.float32)
+    mats = _mats(mats, int(proj_size[2]), "reconstruct_conebeam")
+    nvox = int(vol_size[0]) * int(vol_size[1]) * int(vol_size[2]) * 4
+    nproj = int(proj_size[0]) * int(proj_size[1]) * int(proj_size[2]) * 4
+    _dev_buffer(vol_in, "reconstruct_conebeam vol_in", nvox)
+    _dev_buffer(vol_out, "reconstruct_conebeam vol_out", nvox)
     on_dev = not isinstance(proj, np.ndarray)
     if not on_dev:
         proj = np.ascontiguousarray(proj, np.float32)
+        if proj.nbytes < nproj:
+            raise FdkError("reconstruct_conebeam: host projections smaller than proj_size")
+    else:
+        _dev_buffer(proj, "reconstruct_conebeam proj", nproj)
     pp = _dev_ptr(proj) if on_dev else _np_ptr(proj)
     _check(lib().fdk_reconstruct_conebeam(ps, vs, _np_ptr(mats), _dev_ptr(vol_in),
                                           _dev_ptr(vol_out), pp, 1 if on_dev else 0,
@@ -489,6 +568,9 @@ def reconstruct_conebeam(proj_size, vol_size, mats, vol_in, vol_out, proj, strea
 def CorrectImage_Real32(src, table, dst, stream=None):
     """dst = src * table over a (depth, height, width) device tensor."""
     depth, height, width = src.shape
+    _dev_buffer(src, "CorrectImage src", depth * height * width * 4)
+    _dev_buffer(dst, "CorrectImage dst", depth * height * width * 4)
+    _dev_buffer(table, "CorrectImage table", height * table.shape[-1] * 4)
     _check(lib().fdk_correct_image(_dev_ptr(src), _dev_ptr(table), _dev_ptr(dst), width, height,
                                    depth, width, table.shape[-1], _stream(stream)), "CorrectImage")
 
@@ -497,6 +579,9 @@ def FilterRow_Real32(src, dst, filt, pad_mode: int = PAD_MEAN3, stream=None):
     """Row filter of a (depth, height, width) device tensor with a length-N
     device filter (the cuFFT path pads with the 3-sample mean)."""
     depth, height, width = src.shape
+    _dev_buffer(src, "FilterRow src", depth * height * width * 4)
+    _dev_buffer(dst, "FilterRow dst", depth * height * width * 4)
+    _dev_buffer(filt, "FilterRow filter", 0)
     _check(lib().fdk_filter_rows(_dev_ptr(src), _dev_ptr(dst), width, height, depth, width, width,
                                  _dev_ptr(filt), int(filt.numel()), int(pad_mode),
                                  _stream(stream)), "FilterRow")
@@ -512,12 +597,22 @@ def filter_views(src, dst, cos_table, filt, pad_mode: int = PAD_MEAN4, transpose
     nv, nu = src.shape[-2], src.shape[-1]
     if nviews is None:
         nviews = (src.shape[0] - src_view_offset + src_view_step - 1) // src_view_step
-    n = nviews
+    n = int(nviews)
+    if dtype not in (FDK_F32, FDK_F16, FDK_BF16):
+        raise FdkError(f"filter_views: unknown dtype {dtype}")
+    if n < 0 or src_view_offset < 0 or dst_view_offset < 0 or src_view_step < 1:
+        raise FdkError("filter_views: negative view count or offset, or a step below 1")
     view = nu * nv
     esize = 4 if dtype == FDK_F32 else 2
+    src_need = ((src_view_offset + (n - 1) * src_view_step) * view + view) * 4 if n else 0
+    dst_need = (dst_view_offset + n) * view * esize if n else 0
+    sp = C.c_void_p(_dev_buffer(src, "filter_views src", src_need) + src_view_offset * view * 4)
+    dp = C.c_void_p(_dev_buffer(dst, "filter_views dst", dst_need, dtype) +
+                    dst_view_offset * view * esize)
+    if cos_table is not None:
+        _dev_buffer(cos_table, "filter_views cos table", view * 4)
+    _dev_buffer(filt, "filter_views filter", 0)
     so = np.zeros((max(n, 1), 2), np.float32) if dtype != FDK_F32 else None
-    sp = C.c_void_p(src.data_ptr() + src_view_offset * view * 4)
-    dp = C.c_void_p(dst.data_ptr() + dst_view_offset * view * esize)
     _check(lib().fdk_filter_views(sp, view * int(src_view_step), dp, view, int(dtype), nu, nv, int(n),
                                   _dev_ptr(cos_table) if cos_table is not None else None,
                                   _dev_ptr(filt), int(filt.numel()), int(pad_mode),
@@ -528,15 +623,22 @@ def filter_views(src, dst, cos_table, filt, pad_mode: int = PAD_MEAN4, transpose
 
 
 def FilterProjImage(para: Parameter, proj: np.ndarray, DEPTH: int = 32,
-                    pad_mode: int = PAD_MEAN3) -> np.ndarray:
-    """Host views (nviews, nv, nu) filtered in place (cudaFftFilter.cu:394-452)."""
-    assert proj.dtype == np.float32 and proj.flags.c_contiguous
+                    pad_mode: int = PAD_MEAN3) -> float:
+    """Host views (nviews, nv, nu) filtered in place (cudaFftFilter.cu:394-452);
+    returns the summed milliseconds of the batches' upload, filter and
+    download, as the reference does."""
+    if not isinstance(proj, np.ndarray) or proj.dtype != np.float32 or not proj.flags.c_contiguous:
+        raise FdkError("FilterProjImage: proj must be a C-contiguous float32 numpy array")
+    if proj.ndim != 3 or proj.shape[1:] != (para.nv, para.nu):
+        raise FdkError(f"FilterProjImage: proj shape {proj.shape} is not (views, {para.nv}, "
+                       f"{para.nu})")
     cos = para.cos_table()
     filt = para.filter().astype(np.float32)
+    ms = C.c_float(0.0)
     _check(lib().fdk_filter_proj_image(_np_ptr(proj), para.nu, para.nv, proj.shape[0],
                                        _np_ptr(cos), _np_ptr(filt), len(filt), int(pad_mode),
-                                       int(DEPTH)), "FilterProjImage")
-    return proj
+                                       int(DEPTH), C.byref(ms)), "FilterProjImage")
+    return float(ms.value)
 
 
 class FilterEngine:
@@ -570,6 +672,8 @@ def forward_project(para: Parameter, vol, first: int, count: int, out, step_mm: 
     """Ray-march views first..first+count of ``para`` through device volume
     ``vol`` (nz, ny, nx) into device ``out`` (count, nv, nu)."""
     g = para.c()
+    _dev_buffer(vol, "forward_project vol", para.nx * para.ny * para.nz * 4)
+    _dev_buffer(out, "forward_project out", max(int(count), 0) * para.nu * para.nv * 4)
     _check(lib().fdk_fp_project(C.byref(g), _dev_ptr(vol), int(first), int(count), float(step_mm),
                                 int(mode), _dev_ptr(out), _stream(stream)), "forward_project")
 
@@ -588,12 +692,14 @@ def _ell_array(ell):
 
 def phantom_rasterize(ell, para: Parameter, vol, stream=None):
     g = para.c()
+    _dev_buffer(vol, "phantom_rasterize vol", para.nx * para.ny * para.nz * 4)
     _check(lib().fdk_phantom_rasterize(_ell_array(ell), len(ell), C.byref(g), _dev_ptr(vol),
                                        _stream(stream)), "phantom_rasterize")
 
 
 def phantom_project(ell, para: Parameter, first: int, count: int, out, stream=None):
     g = para.c()
+    _dev_buffer(out, "phantom_project out", max(int(count), 0) * para.nu * para.nv * 4)
     _check(lib().fdk_phantom_project(_ell_array(ell), len(ell), C.byref(g), int(first), int(count),
                                      _dev_ptr(out), _stream(stream)), "phantom_project")
 

@@ -38,3 +38,41 @@ extern "C" int fdk_device_count(void) {
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
 }
+
+// Device buffers for the C++ layer's DevData / DevArray3D (cudaBP/cudaLib.cuh:298-420,
+// 552-640): linear allocations on the current device, copies in either direction.
+extern "C" int fdk_dev_malloc(void** out, size_t bytes) {
+  FDK_REQUIRE(out, "null output pointer");
+  *out = nullptr;
+  if (bytes == 0) return FDK_OK;
+  FDK_HIP(hipMalloc(out, bytes));
+  return FDK_OK;
+}
+
+extern "C" int fdk_dev_free(void* p) {
+  if (p) FDK_HIP(hipFree(p));
+  return FDK_OK;
+}
+
+extern "C" int fdk_memcpy(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return FDK_OK;
+  FDK_REQUIRE(dst && src, "null pointer");
+  FDK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+  return FDK_OK;
+}
+
+extern "C" int fdk_memcpy_2d(void* dst, size_t dst_pitch, const void* src, size_t src_pitch,
+                             size_t width_bytes, size_t rows) {
+  if (width_bytes == 0 || rows == 0) return FDK_OK;
+  FDK_REQUIRE(dst && src, "null pointer");
+  FDK_REQUIRE(dst_pitch >= width_bytes && src_pitch >= width_bytes, "pitch below row width");
+  FDK_HIP(hipMemcpy2D(dst, dst_pitch, src, src_pitch, width_bytes, rows, hipMemcpyDefault));
+  return FDK_OK;
+}
+
+extern "C" int fdk_dev_memset(void* p, int value, size_t bytes) {
+  if (bytes == 0) return FDK_OK;
+  FDK_REQUIRE(p, "null pointer");
+  FDK_HIP(hipMemset(p, value, bytes));
+  return FDK_OK;
+}

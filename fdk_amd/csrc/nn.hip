@@ -1168,37 +1168,63 @@ __global__ void box_copy_kernel(const float* src, int sD, int sH, int sW, int sz
 }
 
 // Photon noise on line integrals: counts ~ Poisson(I0 exp(-mu p)),
-// p <- -ln(max(counts, 1) / I0) / mu.  Counter-based hash RNG per (seed, i);
-// Poisson by inversion below 64 expected counts, normal approximation above.
+// p <- -ln(max(counts, 1) / I0) / mu.  Counter-based hash RNG per (seed, i, draw),
+// so a seed reproduces every pixel whatever the launch shape.  The counts are
+// exact Poisson variates: below 10 expected counts by multiplying uniforms
+// until the product falls under e^-lambda (Knuth's method), from 10 up by
+// W. Hoermann's transformed rejection with squeeze, PTRS ("The transformed
+// rejection method for generating Poisson random variables", Insurance:
+// Mathematics and Economics 12, 1993), with the acceptance test in FP64 (at
+// lambda = 1e5 its log terms are ~1e6 and cancel to O(1)).
 __device__ inline unsigned hash32(unsigned x) {
   x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
   return x;
 }
-__device__ inline float unif(unsigned seed, unsigned long long i, unsigned k) {
+// Uniform in (0, 1) from 32 hashed bits: draw k of pixel i.
+__device__ inline double unif(unsigned seed, unsigned long long i, unsigned k) {
   const unsigned h = hash32(hash32(seed ^ (unsigned)(i * 0x9E3779B97F4A7C15ULL >> 32)) ^
                             hash32((unsigned)i + 0x632BE5ABu * k));
-  return ((h >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  return ((double)h + 0.5) * (1.0 / 4294967296.0);
 }
 
-__global__ void poisson_kernel(float* p, long long n, float i0, float mu, unsigned seed) {
+__device__ double poisson_small(double lam, unsigned seed, unsigned long long i) {
+  const double L = exp(-lam);
+  double prod = unif(seed, i, 0);
+  int c = 0;
+  while (prod > L && c < 1000) {  // P(k > 1000) is 0 in double for lambda < 10
+    ++c;
+    prod *= unif(seed, i, (unsigned)c);
+  }
+  return (double)c;
+}
+
+__device__ double poisson_ptrs(double lam, unsigned seed, unsigned long long i) {
+  const double slam = sqrt(lam), loglam = log(lam);
+  const double b = 0.931 + 2.53 * slam;
+  const double a = -0.059 + 0.02483 * b;
+  const double log_inv_alpha = log(1.1239 + 1.1328 / (b - 3.4));
+  const double vr = 0.9277 - 3.6224 / (b - 2.0);
+  double k = floor(lam);
+  for (unsigned t = 0; t < 128; ++t) {  // each try accepts with probability > 0.9
+    const double U = unif(seed, i, 2 * t + 1) - 0.5;
+    const double V = unif(seed, i, 2 * t + 2);
+    const double us = 0.5 - fabs(U);
+    k = floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return k;  // squeeze: inside the hat's core
+    if (k < 0.0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log_inv_alpha - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0))
+      return k;
+  }
+  return fmax(k, 0.0);
+}
+
+__global__ void poisson_kernel(float* p, long long n, double i0, double mu, unsigned seed) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    const float lam = i0 * expf(-mu * p[i]);
-    float k;
-    if (lam < 64.f) {
-      const float L = expf(-lam);
-      float prod = unif(seed, i, 0);
-      int c = 0;
-      while (prod > L && c < 1000) {
-        ++c;
-        prod *= unif(seed, i, (unsigned)c);
-      }
-      k = (float)c;
-    } else {
-      const float u1 = unif(seed, i, 1), u2 = unif(seed, i, 2);
-      const float g = sqrtf(-2.f * logf(u1)) * cosf(6.28318530718f * u2);
-      k = rintf(lam + sqrtf(lam) * g);
-    }
-    p[i] = -logf(fmaxf(k, 1.f) / i0) / mu;
+    const double lam = i0 * exp(-mu * (double)p[i]);
+    if (!(lam >= 0.0 && lam < 1e15)) continue;  // NaN or absurd counts: left as they are
+    const double k = lam < 10.0 ? poisson_small(lam, seed, (unsigned long long)i)
+                                : poisson_ptrs(lam, seed, (unsigned long long)i);
+    p[i] = (float)(-log(fmax(k, 1.0) / i0) / mu);
   }
 }
 
@@ -1575,8 +1601,10 @@ extern "C" int fdk_nn_box_copy(const float* src, const int src_dims[3], const in
 extern "C" int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t seed,
                                  void* stream) {
   FDK_REQUIRE(dev_proj && i0 > 0 && mu > 0, "bad noise arguments");
+  FDK_REQUIRE(n >= 0, "negative count");
+  if (n == 0) return FDK_OK;
   hipLaunchKernelGGL(poisson_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     dev_proj, (long long)n, (float)i0, (float)mu, seed);
+                     dev_proj, (long long)n, i0, mu, seed);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

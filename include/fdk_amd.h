@@ -92,6 +92,16 @@ int fdk_version(void);
 /* Number of HIP devices visible (0 when none); never fails. */
 int fdk_device_count(void);
 
+/* Device memory for the C++ layer's DevData / DevArray3D (cudaBP/cudaLib.cuh:298-420,
+ * 552-640): linear allocations on the current device; copies use unified
+ * addressing (host or device pointers on either side) and are synchronous. */
+int fdk_dev_malloc(void** out, size_t bytes);
+int fdk_dev_free(void* p);
+int fdk_memcpy(void* dst, const void* src, size_t bytes);
+int fdk_memcpy_2d(void* dst, size_t dst_pitch, const void* src, size_t src_pitch,
+                  size_t width_bytes, size_t rows);
+int fdk_dev_memset(void* p, int value, size_t bytes);
+
 /* ------------------------------------------------------------ geometry (host, FP64) */
 /* type.h:4736 */
 int fdk_geom_angular_ramp_factor(const fdk_geometry* g, double* out);
@@ -193,9 +203,12 @@ int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst
                      const float* dev_cos, const float* dev_filter, int N, int pad_mode,
                      int transpose, float* host_scale_offset, void* stream);
 /* FilterProjImage: host views (nviews x nv x nu) filtered in place, in batches of
- * depth views (32 by default when depth <= 0), cos weighting by host_cos. */
+ * depth views (32 by default when depth <= 0), cos weighting by host_cos.
+ * *elapsed_ms (when not NULL) receives the summed time of the batches' upload,
+ * filter and download, as the reference's return value (cudaFftFilter.cu:416-441). */
 int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nviews, const float* host_cos,
-                          const float* host_filter, int N, int pad_mode, int depth);
+                          const float* host_filter, int N, int pad_mode, int depth,
+                          float* elapsed_ms);
 
 /* FilterEngine::Filtering on one host image (nv rows of nu): src is overwritten
  * with the cos-weighted image (FilterLib.h:181), dst receives the filtered
@@ -317,8 +330,9 @@ int fdk_nn_convert(const void* src, void* dst, int64_t n, int to_bf16, void* str
 int fdk_nn_box_copy(const float* src, const int src_dims[3], const int src_org[3], float* dst,
                     const int dst_dims[3], const int dst_org[3], const int box[3], float scale,
                     void* stream);
-/* Photon noise on line integrals p: counts ~ Poisson(i0 exp(-mu p)), then
- * p = -ln(max(counts, 1) / i0) / mu. */
+/* Photon noise on line integrals p: counts ~ Poisson(i0 exp(-mu p)) (exact
+ * variates: multiplication method below 10 counts, Hoermann's PTRS above), then
+ * p = -ln(max(counts, 1) / i0) / mu.  Deterministic per (seed, element index). */
 int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t seed,
                       void* stream);
 /* D (16x16) = A (16x32) B (32x16) through one v_mfma_f32_16x16x32_bf16 (row-major

@@ -6,11 +6,12 @@
 //   DataRef, FilterRow_Real32, CorrectImage_Real32, FilterProjImage
 //                               FilterlibCufft/cudaBackProjection.cuh:121-229,
 //                               FilterlibCufft/cudaFftFilter.cuh:7-9, cudaFftFilter.cu:394
-//   FrameT, FilterEngine        common/type.h:1893-2005, Filterlib/FilterLib.h:148-203
-//   Parameter                   common/type.h:4627-4895 (fields and the tables of init())
-//   cudaCTForwardProjection_f32 genProj/cudaIterType.cuh:280 (core arguments only)
-//   ProjectionImageT::LoadFolder/SaveToFolder, VolumeT::Save/Load
-//                               common/type.h:1796-1828, 3831-3858, 3256-3310
+//   FrameT, ImageT, FilterEngine common/type.h:1893-2005, 2010-2115, Filterlib/FilterLib.h:148-203
+//   Parameter                   common/type.h:4627-4895 (fields, init() tables, GetProjPath)
+//   DevData, DevArray3D         cudaBP/cudaLib.cuh:298-420, 552-640 (linear device memory)
+//   cudaCTForwardProjection_f32 genProj/cudaIterType.cuh:280, cudaForwardProjection.cu:374-489
+//   ProjectionImageT, VolumeT   common/type.h:3650-3960 (frames, LoadFolder/SaveToFolder,
+//                               Transpose), 2440-2480, 3256-3325 (Save/Load, SetUint)
 //   CheckImageQuality, NCC      common/type.h:4991-5030, 4960-4987
 //
 // Failures throw std::string, as the reference's VERIFY_TRUE / CUDA_CHECK_ERROR
@@ -21,11 +22,14 @@
 #ifndef FDK_AMD_HPP
 #define FDK_AMD_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <sys/stat.h>
@@ -198,10 +202,157 @@ inline bool CorrectImage_Real32(const DataRef<float>& src, const DataRef<float>&
 
 template <typename D>
 struct FrameT {
+  typedef D DataType;
   FrameT() : buffer(nullptr), width(0), height(0), index(-1) {}
   FrameT(D* data, int w, int h, int i) : buffer(data), width(w), height(h), index(i) {}
   D* buffer;
   int width, height, index;
+};
+
+// A host image of width x height samples, x fastest (ImageT, type.h:2010-2115).
+// Copies are deep; `buffer` points at the image's own storage.
+template <typename T>
+struct ImageT {
+  typedef T DataType;
+  ImageT() : width(0), height(0), buffer(nullptr) {}
+  ImageT(int w, int h) : ImageT() { MallocBuffer(w, h); }
+  explicit ImageT(const FrameT<T>& f) : ImageT() {
+    MallocBuffer(f.width, f.height);
+    std::copy(f.buffer, f.buffer + mem.size(), buffer);
+  }
+  template <typename A>
+  ImageT(const A* data, int w, int h) : ImageT() {
+    MallocBuffer(w, h);
+    for (size_t i = 0; i < mem.size(); ++i) buffer[i] = (T)data[i];
+  }
+  ImageT(const ImageT& o) : ImageT() { *this = o; }
+  ImageT& operator=(const ImageT& o) {
+    if (this != &o) {
+      mem = o.mem;
+      width = o.width;
+      height = o.height;
+      buffer = mem.empty() ? nullptr : mem.data();
+    }
+    return *this;
+  }
+  // Same size keeps the samples; another size reallocates zeroed storage.
+  ImageT& MallocBuffer(int w, int h) {
+    if (w < 0 || h < 0) throw std::string("ImageT::MallocBuffer: negative size");
+    if (w != width || h != height || !buffer) {
+      mem.assign((size_t)w * h, T());
+      width = w;
+      height = h;
+      buffer = mem.empty() ? nullptr : mem.data();
+    }
+    return *this;
+  }
+  FrameT<T> GetFrame() const { return FrameT<T>(buffer, width, height, 0); }
+  T& operator()(int x, int y) { return buffer[(size_t)y * width + x]; }
+  const T& operator()(int x, int y) const { return buffer[(size_t)y * width + x]; }
+  ImageT& Zero() {
+    std::fill(mem.begin(), mem.end(), T());
+    return *this;
+  }
+  ImageT& Clear() { return Zero(); }
+  int width, height;
+  T* buffer;
+
+ private:
+  std::vector<T> mem;
+};
+
+// Device buffer of width x height x depth samples, rows packed (DataPitch() ==
+// width), on the current device (DevData, cudaBP/cudaLib.cuh:298-420).
+// Copies between host and device are synchronous; copies of the object are deep.
+template <typename T>
+struct DevData {
+  typedef T DataType;
+  DevData() {}
+  explicit DevData(size_t w, size_t h = 1, size_t d = 1) { MallocBuffer(w, h, d); }
+  DevData(const DevData& o) { *this = o; }
+  DevData(DevData&& o) noexcept { swap(o); }
+  DevData& operator=(const DevData& o) {
+    if (this != &o) {
+      MallocBuffer(o.width, o.height, o.depth);
+      fdk_amd_detail::check(fdk_memcpy(ptr, o.ptr, bytes()), "DevData copy");
+    }
+    return *this;
+  }
+  DevData& operator=(DevData&& o) noexcept {
+    swap(o);
+    return *this;
+  }
+  virtual ~DevData() { FreeBuffer(); }
+  DevData& MallocBuffer(size_t w, size_t h, size_t d) {
+    if (w == width && h == height && d == depth && (ptr || w * h * d == 0)) return *this;
+    FreeBuffer();
+    void* p = nullptr;
+    fdk_amd_detail::check(fdk_dev_malloc(&p, sizeof(T) * w * h * d), "DevData::MallocBuffer");
+    ptr = static_cast<T*>(p);
+    width = w;
+    height = h;
+    depth = d;
+    channels = d == 1 ? (h == 1 ? 1 : 2) : 3;
+    return *this;
+  }
+  DevData& MallocBuffer(size_t w, size_t h) { return MallocBuffer(w, h, 1); }
+  DevData& MallocBuffer(size_t w) { return MallocBuffer(w, 1, 1); }
+  void FreeBuffer() {
+    if (ptr) fdk_dev_free(ptr);
+    ptr = nullptr;
+    width = height = depth = channels = 0;
+  }
+  // Host rows of _w samples, _pitch samples apart, _h * _d rows.
+  void CopyFromHost(const T* host, size_t _w, size_t _pitch, size_t _h, size_t _d = 1) {
+    copy_rows(ptr, width, host, _pitch, _w, _h * _d, "DevData::CopyFromHost");
+  }
+  void CopyToHost(T* host, size_t _w, size_t _pitch, size_t _h, size_t _d = 1) const {
+    copy_rows(host, _pitch, ptr, width, _w, _h * _d, "DevData::CopyToHost");
+  }
+  DevData& Zero() {
+    fdk_amd_detail::check(fdk_dev_memset(ptr, 0, bytes()), "DevData::Zero");
+    return *this;
+  }
+  size_t DataPitch() const { return width; }
+  T* GetData() const { return ptr; }
+  size_t width = 0, height = 0, depth = 0, channels = 0;
+
+ protected:
+  size_t bytes() const { return sizeof(T) * width * height * depth; }
+  void copy_rows(T* dst, size_t dpitch, const T* src, size_t spitch, size_t w, size_t rows,
+                 const char* what) const {
+    if (w > width || rows > height * depth)
+      throw std::string(what) + ": copy larger than the buffer";
+    fdk_amd_detail::check(fdk_memcpy_2d(dst, dpitch * sizeof(T), src, spitch * sizeof(T),
+                                        w * sizeof(T), rows),
+                          what);
+  }
+  void swap(DevData& o) noexcept {
+    std::swap(ptr, o.ptr);
+    std::swap(width, o.width);
+    std::swap(height, o.height);
+    std::swap(depth, o.depth);
+    std::swap(channels, o.channels);
+  }
+  T* ptr = nullptr;
+};
+
+// Device volume of width x height x depth samples, x fastest (DevArray3D,
+// cudaBP/cudaLib.cuh:552-640).  Linear memory: the kernels gather through
+// buffer loads, not texture units, so the cudaArray flags are accepted and
+// ignored.
+template <typename T>
+struct DevArray3D : public DevData<T> {
+  DevArray3D(unsigned int flags = 0) { (void)flags; }
+  DevArray3D(size_t w, size_t h, size_t d, unsigned int flags = 0) {
+    (void)flags;
+    this->MallocBuffer(w, h, d);
+  }
+  DevArray3D& MallocBuffer(size_t w, size_t h, size_t d, unsigned int flags = 0) {
+    (void)flags;
+    DevData<T>::MallocBuffer(w, h, d);
+    return *this;
+  }
 };
 
 // Scan geometry and the tables Parameter::init builds (type.h:4719-4754).
@@ -219,14 +370,32 @@ struct Parameter {
   int64_t nProj = 512;
   double volume_postion_angle = -3.141592653589793238462 * 48.8671875 / 180.0;
   int step = 1;
+  int compute_device = 1;
+  int64_t verbose = 0, nBatchCount = 0;
   std::string str_filter = "shepp-logan";
+  std::string str_main_folder, str_proj_img_folder, str_working_folder, str_mask_folder;
 
+  // Derived by init().
+  double sx = 0, sy = 0, sz = 0, su = 0, sv = 0;  // extents in mm
+  double offset_su = 0, offset_sv = 0;            // detector offsets in mm
   double AngularAndrampFactor = 0.0;
-  std::vector<float> cos_sigma;             // nv x nu, u fastest
+  struct {
+    ImageT<float> cos_sigma, csc_sigma;  // nu x nv (u fastest): weight and 1 / weight
+  } cos_image_table;
   std::vector<double> fft_kernel_real;      // N
   std::vector<float> fft_kernel_real_f32;   // N
   std::vector<float> vec_rot_mat_ijk3x4;    // nProj x 12
   std::vector<double> vec_deg;
+
+  // Path of view `index` of the scan: <str_proj_img_folder>img%04d.raw numbered
+  // index * step (type.h:4852-4856).
+  std::string GetProjPath(int index) const {
+    if (index < 0 || index >= nProj)
+      throw std::string("Parameter::GetProjPath: index outside 0..nProj-1");
+    char name[32];
+    std::snprintf(name, sizeof(name), "img%04d.raw", index * step);
+    return str_proj_img_folder + name;
+  }
 
   fdk_geometry geometry() const {
     static const char* names[6] = {"none", "ram-lak", "shepp-logan", "cosine", "hamming", "hann"};
@@ -252,9 +421,20 @@ struct Parameter {
 
   void init() {
     const fdk_geometry g = geometry();
+    sx = dx * nx;
+    sy = dy * ny;
+    sz = dz * nz;
+    su = du * nu;
+    sv = dv * nv;
+    offset_su = offset_nu * du;
+    offset_sv = offset_nv * dv;
     fdk_amd_detail::check(fdk_geom_angular_ramp_factor(&g, &AngularAndrampFactor), "Parameter");
-    cos_sigma.resize((size_t)nu * nv);
-    fdk_amd_detail::check(fdk_geom_cos_table(&g, cos_sigma.data()), "CalculateCosTable");
+    ImageT<float>& cs = cos_image_table.cos_sigma;
+    ImageT<float>& csc = cos_image_table.csc_sigma;
+    cs.MallocBuffer((int)nu, (int)nv);
+    csc.MallocBuffer((int)nu, (int)nv);
+    fdk_amd_detail::check(fdk_geom_cos_table(&g, cs.buffer), "CalculateCosTable");
+    for (size_t i = 0; i < (size_t)nu * nv; ++i) csc.buffer[i] = (float)(1.0 / cs.buffer[i]);
     const int n = fdk_geom_filter_length(&g);
     fdk_amd_detail::check(n, "filter length");
     fft_kernel_real.resize(n);
@@ -268,18 +448,58 @@ struct Parameter {
   }
 };
 
-// A stack of projection images in host memory: count frames of width x height.
+// A stack of projection images in host memory: count frames of width x height
+// (ProjectionImageT, type.h:3650-3960).  As in the reference, frames of a const
+// stack are writable (FilterProjImage filters a const reference in place).
 template <typename T>
 struct ProjectionImageT {
+  typedef T DataType;
+  typedef FrameT<T> FrameType;
   ProjectionImageT() : width(0), height(0), count(0) {}
   ProjectionImageT(int w, int h, int c) : width(w), height(h), count(c), data((size_t)w * h * c) {}
   int GetCount() const { return count; }
-  bool GetFrame(FrameT<T>& f, int i) {
+  int GetStep() const { return step; }
+  ProjectionImageT& SetStep(int s) {
+    step = s;
+    return *this;
+  }
+  // Same shape keeps the samples; another shape reallocates zeroed frames.
+  ProjectionImageT& MallocBuffer(int64_t w, int64_t h, int64_t c, int _step = 1) {
+    if (w < 0 || h < 0 || c < 0) throw std::string("ProjectionImageT::MallocBuffer: negative size");
+    if (w != width || h != height || c != count) *this = ProjectionImageT((int)w, (int)h, (int)c);
+    step = _step;
+    return *this;
+  }
+  bool IsValid() const { return !data.empty() && count > 0; }
+  T* Buffer() const { return const_cast<T*>(data.data()); }
+  int64_t BufferSize() const { return (int64_t)width * height * count; }
+  bool GetFrame(FrameT<T>& f, int i) const {
     if (i < 0 || i >= count) return false;
-    f = FrameT<T>(data.data() + (size_t)width * height * i, width, height, i);
+    f = FrameT<T>(Buffer() + (size_t)width * height * i, width, height, i);
     return true;
   }
-  T* buffer() { return data.data(); }
+  FrameT<T> GetFrame(int i) const {
+    FrameT<T> f;
+    if (!GetFrame(f, i)) throw std::string("ProjectionImageT::GetFrame: index outside the stack");
+    return f;
+  }
+  bool SetFrame(const T* src, int w, int h, int i) {
+    if (w != width || h != height || i < 0 || i >= count) return false;
+    std::copy(src, src + (size_t)w * h, data.begin() + (size_t)w * h * i);
+    return true;
+  }
+  // Every frame transposed in place; width and height swap (type.h:3936-3958).
+  ProjectionImageT& Transpose() {
+    std::vector<T> tmp((size_t)width * height);
+    for (int i = 0; i < count; ++i) {
+      T* f = data.data() + (size_t)width * height * i;
+      for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) tmp[(size_t)height * x + y] = f[(size_t)y * width + x];
+      std::copy(tmp.begin(), tmp.end(), f);
+    }
+    std::swap(width, height);
+    return *this;
+  }
 
   // LoadFolder (type.h:3831-3844 with LoadProjectionImageFolder, 1796-1828):
   // frame k*step <- <folder>img%04d.raw numbered k*step + start_index, each
@@ -311,6 +531,7 @@ struct ProjectionImageT {
       T* dst = data.data() + n * k * step;
       for (size_t e = 0; e < n; ++e) dst[e] = (T)tmp[e];
     }
+    this->step = step;
     return true;
   }
   // SaveToFolder (type.h:3845-3858): frames 0, step, 2*step, ... as
@@ -334,6 +555,7 @@ struct ProjectionImageT {
   }
 
   int width, height, count;
+  int step = 1;
   std::vector<T> data;
 };
 typedef ProjectionImageT<float> ProjectionImage;
@@ -343,15 +565,47 @@ typedef ProjectionImageT<float> ProjectionImage;
 // the samples slice by slice.
 template <typename T>
 struct VolumeT {
-  VolumeT() : nx(0), ny(0), nz(0) {}
-  VolumeT(int x, int y, int z) : nx(x), ny(y), nz(z), data((size_t)x * y * z) {}
-  T* buffer() { return data.data(); }
-  const T* buffer() const { return data.data(); }
+  typedef T DataType;
+  VolumeT() : nx(0), ny(0), nz(0), buffer(nullptr) {}
+  VolumeT(int64_t x, int64_t y, int64_t z) : VolumeT() { MallocBuffer(x, y, z); }
+  VolumeT(const VolumeT& o) : VolumeT() { *this = o; }
+  VolumeT& operator=(const VolumeT& o) {
+    if (this != &o) {
+      data = o.data;
+      nx = o.nx;
+      ny = o.ny;
+      nz = o.nz;
+      dx = o.dx;
+      dy = o.dy;
+      dz = o.dz;
+      buffer = data.empty() ? nullptr : data.data();
+    }
+    return *this;
+  }
+  // Same size keeps the samples; another size reallocates zeroed storage.
+  VolumeT& MallocBuffer(int64_t x, int64_t y, int64_t z) {
+    if (x < 0 || y < 0 || z < 0) throw std::string("VolumeT::MallocBuffer: negative size");
+    if (x != nx || y != ny || z != nz || !buffer) {
+      data.assign((size_t)(x * y * z), T());
+      nx = x;
+      ny = y;
+      nz = z;
+      buffer = data.empty() ? nullptr : data.data();
+    }
+    return *this;
+  }
+  int64_t BufferSize() const { return nx * ny * nz; }
+  // Voxel spacing in mm (VolumeT::SetUint, type.h:3321-3325).
+  void SetUint(float _dx, float _dy, float _dz) {
+    dx = _dx;
+    dy = _dy;
+    dz = _dz;
+  }
   bool Save(const std::string& path) const {
     fdk_amd_detail::make_dirs(path);
     std::FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return false;
-    const int32_t hdr[3] = {nx, ny, nz};
+    const int32_t hdr[3] = {(int32_t)nx, (int32_t)ny, (int32_t)nz};
     bool ok = std::fwrite(hdr, sizeof(int32_t), 3, f) == 3;
     ok = ok && std::fwrite(data.data(), sizeof(T), data.size(), f) == data.size();
     std::fclose(f);
@@ -364,7 +618,7 @@ struct VolumeT {
     bool ok = std::fread(hdr, sizeof(int32_t), 3, f) == 3 && hdr[0] > 0 && hdr[1] > 0 &&
               hdr[2] > 0;
     if (ok) {
-      *this = VolumeT(hdr[0], hdr[1], hdr[2]);
+      MallocBuffer(hdr[0], hdr[1], hdr[2]);
       ok = std::fread(data.data(), sizeof(T), data.size(), f) == data.size();
     }
     std::fclose(f);
@@ -386,22 +640,29 @@ struct VolumeT {
     }
     return true;
   }
-  int32_t nx, ny, nz;
+  int64_t nx, ny, nz;
+  float dx = 0.f, dy = 0.f, dz = 0.f;
+  T* buffer;  // the samples, nz x ny x nx with x fastest
   std::vector<T> data;
 };
+typedef VolumeT<float> Volume;
 
-// FilterProjImage: views filtered in place, DEPTH views per device batch;
-// returns the elapsed milliseconds as the reference does.
-inline float FilterProjImage(const Parameter& para, ProjectionImage& proj_img, int DEPTH = 32) {
+// FilterProjImage (FilterlibCufft/cudaFftFilter.cu:394-452): views filtered in
+// place, DEPTH views per device batch, with the cuFFT path's 3-sample pad;
+// returns the summed milliseconds of the batches (upload, cos weighting and
+// filter, download), as the reference does.  The reference takes the stack by
+// const reference and writes its frames; so does this.
+inline float FilterProjImage(const Parameter& para, const ProjectionImage& proj_img,
+                             int DEPTH = 32) {
   if (proj_img.width != para.nu || proj_img.height != para.nv)
     throw std::string("FilterProjImage: image size differs from the parameter");
-  fdk_amd_detail::check(fdk_filter_proj_image(proj_img.buffer(), (int)para.nu, (int)para.nv,
-                                              proj_img.GetCount(), para.cos_sigma.data(),
-                                              para.fft_kernel_real_f32.data(),
-                                              (int)para.fft_kernel_real_f32.size(), FDK_PAD_MEAN3,
-                                              DEPTH),
-                        "FilterProjImage");
-  return 0.0f;
+  float ms = 0.f;
+  fdk_amd_detail::check(
+      fdk_filter_proj_image(proj_img.Buffer(), (int)para.nu, (int)para.nv, proj_img.GetCount(),
+                            para.cos_image_table.cos_sigma.buffer, para.fft_kernel_real_f32.data(),
+                            (int)para.fft_kernel_real_f32.size(), FDK_PAD_MEAN3, DEPTH, &ms),
+      "FilterProjImage");
+  return ms;
 }
 
 // FilterEngine (Filterlib/FilterLib.h:148-203): cos weighting (src is
@@ -437,9 +698,99 @@ inline bool cudaCTForwardProjection_f32(const Parameter& para, const std::vector
   if (out.width != para.nu || out.height != para.nv || out.count < count)
     throw std::string("cudaCTForwardProjection_f32: output size");
   fdk_amd_detail::check(fdk_fp_project_host(&g, volume.data(), first, count, step_mm,
-                                            FDK_FP_CLAMP, out.buffer()),
+                                            FDK_FP_CLAMP, out.Buffer()),
                         "cudaCTForwardProjection_f32");
   return true;
+}
+
+// cudaCTForwardProjection_f32 with the reference's arguments
+// (genProj/cudaIterType.cuh:280, cudaForwardProjection.cu:374-489).
+//  * The sampled volume is the device volume pDevVolRoi (the reference binds it
+//    to the texture its projector reads); when it is NULL the host `volume` is
+//    uploaded.  Sizes and spacing come from `volume` (nx, ny, nz, dx, dy, dz:
+//    VolumeT::SetUint), as the reference's volume geometry does.
+//  * pDevVolMask is accepted and not read: the reference binds it, and its
+//    single-volume projector (kernel_forwardProject) never samples it.
+//  * proj_index >= 0: that view is projected into dev_forward_proj (allocated
+//    nu x nv by the caller) and copied into forward_proj.  proj_index < 0: views
+//    0, step, 2 step, ... are projected into (*pvec_dev_forward_proj)[i] when it
+//    is given, and into forward_proj and frame i of *pforward_proj_2th when that
+//    is given.  forward_proj ends holding the last view projected.
+//  * GPU sampling semantics of the reference projector: step min(dx, dy, dz),
+//    t < tfar, clamp-to-edge trilinear.  `proj` only fed the reference's
+//    on-screen comparison and is not read.
+//  * Errors are printed and return false, as the reference's try/catch does.
+inline bool cudaCTForwardProjection_f32(const ProjectionImageT<float>& proj, const Parameter& para,
+                                        const VolumeT<float>& volume,
+                                        DevArray3D<float>* pDevVolRoi,
+                                        DevArray3D<float>* pDevVolMask,
+                                        ImageT<float>& forward_proj,
+                                        DevData<float>& dev_forward_proj,
+                                        std::vector<DevData<float>>* pvec_dev_forward_proj,
+                                        ProjectionImageT<float>* pforward_proj_2th = nullptr,
+                                        int proj_index = -1) {
+  (void)proj;
+  (void)pDevVolMask;
+  try {
+    if (!(volume.dx > 0 && volume.dy > 0 && volume.dz > 0))
+      throw std::string("volume spacing not set (VolumeT::SetUint)");
+    if (forward_proj.width <= 0 || forward_proj.height <= 0)
+      throw std::string("forward_proj has no size");
+    fdk_geometry g = para.geometry();
+    g.nx = (int)volume.nx;
+    g.ny = (int)volume.ny;
+    g.nz = (int)volume.nz;
+    g.dx = volume.dx;
+    g.dy = volume.dy;
+    g.dz = volume.dz;
+    DevArray3D<float> uploaded;
+    const DevData<float>* vol = pDevVolRoi;
+    if (!vol) {
+      uploaded.MallocBuffer(volume.nx, volume.ny, volume.nz);
+      uploaded.CopyFromHost(volume.buffer, volume.nx, volume.nx, volume.ny, volume.nz);
+      vol = &uploaded;
+    }
+    if ((int64_t)vol->width != volume.nx || (int64_t)vol->height != volume.ny ||
+        (int64_t)vol->depth != volume.nz)
+      throw std::string("device volume size differs from the volume");
+    auto project = [&](int i, DevData<float>& dst) {
+      if ((int64_t)(dst.width * dst.height * dst.depth) < para.nu * para.nv)
+        throw std::string("device projection buffer smaller than nu x nv");
+      fdk_amd_detail::check(fdk_fp_project(&g, vol->GetData(), i, 1, 0.0, FDK_FP_CLAMP,
+                                           dst.GetData(), nullptr),
+                            "fdk_fp_project");
+      forward_proj.MallocBuffer((int)para.nu, (int)para.nv);
+      dst.CopyToHost(forward_proj.buffer, para.nu, para.nu, para.nv);
+    };
+    if (proj_index >= 0) {
+      if (proj_index < para.nProj) project(proj_index, dev_forward_proj);  // else no view, as the reference
+      return true;
+    }
+    if (pvec_dev_forward_proj && (int64_t)pvec_dev_forward_proj->size() < para.nProj)
+      throw std::string("fewer device projections than views");
+    for (int i = 0; i < para.nProj; i += para.step) {
+      bool done = false;
+      if (pvec_dev_forward_proj) {
+        DevData<float>& d = (*pvec_dev_forward_proj)[i];
+        if (!d.GetData()) d.MallocBuffer(para.nu, para.nv);
+        project(i, d);
+        done = true;
+      }
+      if (pforward_proj_2th) {
+        if (!done) {
+          if (!dev_forward_proj.GetData()) dev_forward_proj.MallocBuffer(para.nu, para.nv);
+          project(i, dev_forward_proj);
+        }
+        if (!pforward_proj_2th->SetFrame(forward_proj.buffer, forward_proj.width,
+                                         forward_proj.height, i))
+          throw std::string("pforward_proj_2th frame size");
+      }
+    }
+    return true;
+  } catch (const std::string& e) {
+    std::printf("error : cudaCTForwardProjection_f32: %s\n", e.c_str());
+    return false;
+  }
 }
 
 // CheckImageQuality (common/type.h:4991-5030): error per voxel, MSE, PSNR

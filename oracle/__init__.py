@@ -326,3 +326,89 @@ def ref_fp_3dct(g: Geom, vol: np.ndarray, view: int) -> np.ndarray:
     gc = g.c()
     ref().ref_fp_3dct(C.byref(gc), _p(vol), int(view), _p(out))
     return out
+
+
+# ------------------------------------------------------------------ photon noise
+# No reference counterpart (the reference has no noise model; BASELINE's learned
+# configuration asks for Poisson noise at I0 = 1e5).  A numpy restatement of the
+# library's sampler: the same counter-based hash uniforms, Knuth's multiplication
+# method below 10 expected counts, Hoermann's PTRS transformed rejection (1993)
+# from 10 up.  Pinned by the Poisson law itself (tests compare its counts with
+# scipy.stats.poisson: mean, variance, chi-square of the histogram).
+
+def _hash32(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x7FEB352D)
+    x ^= x >> np.uint32(15)
+    x *= np.uint32(0x846CA68B)
+    x ^= x >> np.uint32(16)
+    return x
+
+
+def _noise_unif(seed: int, idx: np.ndarray, k) -> np.ndarray:
+    i64 = idx.astype(np.uint64)
+    hi = ((i64 * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(32)).astype(np.uint32)
+    lo = (i64 & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    kk = np.asarray(k, dtype=np.uint64) * np.uint64(0x632BE5AB)
+    h = _hash32(np.uint32(seed) ^ _hash32(hi)) ^ _hash32(
+        (lo.astype(np.uint64) + kk).astype(np.uint64).astype(np.uint32))
+    return (h.astype(np.float64) + 0.5) * (1.0 / 4294967296.0)
+
+
+def poisson_counts(lam: np.ndarray, seed: int, idx: np.ndarray | None = None) -> np.ndarray:
+    """Poisson variates of mean lam (float64), element i drawn from the hash
+    stream (seed, idx[i]); idx defaults to 0..n-1."""
+    from scipy.special import gammaln
+    lam = np.asarray(lam, np.float64).ravel()
+    idx = np.arange(lam.size, dtype=np.uint64) if idx is None else np.asarray(idx, np.uint64)
+    out = np.zeros(lam.size)
+    small = np.nonzero(lam < 10.0)[0]
+    if small.size:
+        L = np.exp(-lam[small])
+        prod = _noise_unif(seed, idx[small], 0)
+        c = np.zeros(small.size)
+        for t in range(1, 1001):
+            go = prod > L
+            if not go.any():
+                break
+            c[go] += 1
+            prod[go] *= _noise_unif(seed, idx[small][go], t)
+        out[small] = c
+    big = np.nonzero(lam >= 10.0)[0]
+    if big.size:
+        lb = lam[big]
+        slam, loglam = np.sqrt(lb), np.log(lb)
+        b = 0.931 + 2.53 * slam
+        a = -0.059 + 0.02483 * b
+        lia = np.log(1.1239 + 1.1328 / (b - 3.4))
+        vr = 0.9277 - 3.6224 / (b - 2.0)
+        k = np.floor(lb)
+        done = np.zeros(big.size, bool)
+        for t in range(128):
+            todo = np.nonzero(~done)[0]
+            if todo.size == 0:
+                break
+            U = _noise_unif(seed, idx[big][todo], 2 * t + 1) - 0.5
+            V = _noise_unif(seed, idx[big][todo], 2 * t + 2)
+            us = 0.5 - np.abs(U)
+            kt = np.floor((2.0 * a[todo] / us + b[todo]) * U + lb[todo] + 0.43)
+            k[todo] = kt
+            acc = (us >= 0.07) & (V <= vr[todo])
+            rest = ~acc & ~((kt < 0) | ((us < 0.013) & (V > us)))
+            kk = np.where(rest, kt, 0.0)
+            with np.errstate(divide="ignore"):
+                acc2 = rest & (np.log(V) + lia[todo] - np.log(a[todo] / (us * us) + b[todo])
+                               <= -lb[todo] + kk * loglam[todo] - gammaln(kk + 1.0))
+            done[todo[acc | acc2]] = True
+        out[big] = np.maximum(k, 0.0)
+    return out
+
+
+def poisson_noise(p: np.ndarray, i0: float, mu: float, seed: int) -> np.ndarray:
+    """Line integrals p (float32) -> noisy line integrals, as the library's
+    photon-noise step: counts ~ Poisson(i0 exp(-mu p)), -ln(max(counts, 1)/i0)/mu."""
+    p = np.asarray(p, np.float32)
+    lam = i0 * np.exp(-mu * p.astype(np.float64).ravel())
+    k = poisson_counts(lam, seed)
+    return (-np.log(np.maximum(k, 1.0) / i0) / mu).astype(np.float32).reshape(p.shape)

@@ -20,7 +20,7 @@ int main(int argc, char** argv) {
   if (!v.Load(dir + "vol.raw")) return 5;
   double t = 0.0;
   for (float x : v.data) t += x;
-  std::printf("vol %d %d %d sum %.6f\n", v.nx, v.ny, v.nz, t);
+  std::printf("vol %d %d %d sum %.6f\n", (int)v.nx, (int)v.ny, (int)v.nz, t);
   VolumeT<float> w(3, 2, 4);
   for (size_t i = 0; i < w.data.size(); ++i) w.data[i] = 0.5f * (float)i;
   if (!w.Save(dir + "vol_cpp.raw")) return 6;

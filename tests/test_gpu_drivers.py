@@ -58,3 +58,20 @@ def test_cpp_forward_projection_driver_matches_oracle(tmp_path):
     ref = np.stack([O.fp(g, vol, s, 0.5 * g.dx, O.FP_GPU_CLAMP) for s in range(views)])
     close = np.abs(fp - ref) <= 1e-4 * np.abs(ref).max()
     assert close.mean() > 0.995
+
+
+def test_cpp_filter_proj_image_driver_matches_oracle(tmp_path):
+    """FilterlibCufft's test_filter call forms: LoadFolder, FilterProjImage
+    (which returns the summed milliseconds), SaveToFolder."""
+    n, views = 64, 40
+    prefix = str(tmp_path / "filter")
+    run("filter", n, views, prefix)
+    g = geom(n, views)
+    raw = np.stack([np.fromfile(f"{prefix}_raw/img{i:04d}.raw", np.float32).reshape(n, n)
+                    for i in range(views)])
+    flt = np.stack([np.fromfile(f"{prefix}_flt/img{i:04d}.raw", np.float32).reshape(n, n)
+                    for i in range(views)])
+    H = O.filter_design(5, g.nu).astype(np.float32).astype(np.float64)
+    ref = O.filter_rows(O.cos_weight(raw, O.cos_table(g)), H, O.PAD_MEAN3)
+    err = float(np.linalg.norm(flt - ref) / np.linalg.norm(ref))
+    assert err < 2e-6, err

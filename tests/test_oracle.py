@@ -320,3 +320,41 @@ def test_phantom_volume_and_central_chord():
     g2 = O.Geom(nx=8, ny=8, nz=8, nu=65, nv=65, du=1.0, dv=1.0, nproj=4)
     proj = O.phantom_project(s, g2, 0)
     assert proj[32, 32] == pytest.approx(2.0 * 24.0, rel=1e-12)   # through the centre
+
+
+@pytest.mark.parametrize("pad_mode", [O.PAD_MEAN4, O.PAD_MEAN3])
+@pytest.mark.parametrize("nu", [6, 37, 64, 200, 256])
+def test_mean_pads_on_random_rows_match_an_explicit_dft(pad_mode, nu):
+    """Non-constant rows: the oracle's padded row filter equals a numpy DFT of
+    the explicitly padded row.  Filterlib (FilterLib.h:37-46) fills the left
+    (N - nu) / 2 samples with (x0 + x1 + x2 + x3) / 4 and the right ones with
+    (x[-1] + x[-2] + x[-3] + x[-4]) / 4, summed in float; the cuFFT path
+    (cudaFftFilter.cu:22-30) uses the 3-sample means.  Odd nu: the one extra
+    right sample takes the right mean (a deliberate deviation from the
+    reference, whose stale sample is undefined)."""
+    rng = np.random.default_rng(nu + 10 * pad_mode)
+    N = O.nextpow2(nu)
+    H = O.filter_design(5, nu)
+    rows = (rng.standard_normal((3, 4, nu)) + 2.0).astype(np.float32)
+    out = O.filter_rows(rows, H, pad_mode)
+    off = (N - nu) // 2
+    k = 4 if pad_mode == O.PAD_MEAN4 else 3
+    flat = rows.reshape(-1, nu)
+    ref = np.empty_like(flat, dtype=np.float64)
+    for i, x in enumerate(flat):
+        left = np.float32(0.0)
+        right = np.float32(0.0)
+        for j in range(k):  # summed in float, in the reference's order
+            left = np.float32(left + x[j])
+            right = np.float32(right + x[nu - 1 - j])
+        left, right = np.float32(left / np.float32(k)), np.float32(right / np.float32(k))
+        padded = np.empty(N)
+        padded[:off] = left
+        padded[off:off + nu] = x
+        padded[off + nu:] = right
+        ref[i] = np.fft.ifft(np.fft.fft(padded) * H).real[off:off + nu]
+    err = np.abs(out.reshape(-1, nu) - ref).max() / np.abs(ref).max()
+    assert err < 2e-6, err
+    # the pad values matter: the same rows with the other mode differ
+    other = O.filter_rows(rows, H, O.PAD_MEAN3 if pad_mode == O.PAD_MEAN4 else O.PAD_MEAN4)
+    assert np.abs(other.reshape(-1, nu) - ref).max() / np.abs(ref).max() > 1e-4
