@@ -23,6 +23,9 @@ import time
 METRIC = "backprojection GUPS at 2048^3/4096 views; learned-vs-plain recon time @ RMSE"
 REFERENCE_BP_GUPS_V100 = 150.9  # BP GUPS per V100, same problem (reference .t2 run artefacts)
 HBM_PEAK = 8.0e12               # MI355X HBM3E, B/s
+# MI355X vector issue: 256 CUs x 2.4 GHz x one wave64 VALU instruction per CU
+# per clock (four SIMD16 units, four clocks per wave64 instruction)
+VALU_PEAK = 256 * 2.4e9
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -51,6 +54,10 @@ def parse():
                     help="distinct host views, cycled through the scan")
     ap.add_argument("--stream-source", choices=["filtered", "raw"], default="filtered",
                     help="raw: FP32 raw views are streamed and filtered on the GPU")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the streamed-slab leg of the default line")
+    ap.add_argument("--stream-leg-views", type=int, default=256,
+                    help="views streamed in the default line's streamed-slab leg")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned sparse-view comparison (1024^3, 256 vs 1024 views)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -79,21 +86,181 @@ def geometry(n: int, views: int):
                        volume_position_angle_rad=0.0, filter_type=5)
 
 
-def cpu_baseline(p, proj_t_host_fn, seconds: float):
-    """Reference CPU backprojection (3dct BackProjection<float>, compiled from
-    the reference sources into oracle/_ref) timed on this host's cores on the
-    central 256^3 sub-volume of the same geometry; falls back to the oracle
-    port when oracle/_ref is absent."""
+def host_info():
+    """Host cores used and the CPU model (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return {"cores": threads, "nproc": os.cpu_count(), "cpu_model": model}
+
+
+def rel_rmse(a, b) -> float:
+    import numpy as np
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def c1_geometry():
+    import fdk_amd as F
+    return F.Parameter(nx=256, ny=256, nz=256, dx=1.0, dy=1.0, dz=1.0, nu=256, nv=256, du=1.6,
+                       dv=1.6, sod=1000.0, sid=1500.0, nproj=360, volume_position_angle_rad=0.0,
+                       filter_type=5)
+
+
+def c1_gpu():
+    """Config 1 on the GPU: 10-ellipsoid phantom (seed 1), ray-marched
+    projections at a 0.5-voxel step, 360 views; FDK = fused cos weight + mean-4
+    Hann ramp filter + 32-view backprojection passes, timed with events (the
+    projections already in HBM).  Returns the timing, the volume and the raw
+    projections (host) for the CPU leg."""
+    import numpy as np
+    import torch
+    import fdk_amd as F
+    p = c1_geometry()
+    ell = F.random_ellipsoids(10, seed=1, radius_mm=120.0, half_height_mm=120.0)
+    vol = torch.empty((p.nz, p.ny, p.nx), device="cuda")
+    F.phantom_rasterize(ell, p, vol)
+    raw = torch.empty((p.nproj, p.nv, p.nu), device="cuda")
+    F.forward_project(p, vol, 0, p.nproj, raw, 0.5, F.FP_CLAMP)
+    cos = torch.as_tensor(p.cos_table()).cuda()
+    H = torch.as_tensor(p.filter().astype(np.float32)).cuda()
+    store = torch.empty((p.nproj, p.nu, p.nv), device="cuda")
+    mats = p.proj_matrices().astype(np.float32)
+    half = p.nz // 2
+    bp = F.BackProjectionFDK(p.nv, p.nu, 32, p.nx, p.ny, p.nz, (0, 0, 0, p.nx, p.ny, half),
+                             (0, 0, half, p.nx, p.ny, half))
+    st = torch.cuda.current_stream()
+    bp.set_stream(st.cuda_stream)
+
+    def fdk():
+        bp.zero()
+        F.filter_views(raw, store, cos, H, F.PAD_MEAN4, True)
+        for b in range(0, p.nproj, 32):
+            k = min(32, p.nproj - b)
+            bp.bp_device(store, p.nv, p.nu, mats[b:b + k], k, offset_views=b)
+
+    fdk()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record(st)
+    for _ in range(reps):
+        fdk()
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    out = bp.GetVolumeData()
+    bp.close()
+    upd = 256.0 ** 3 * 360
+    return ({"gpu_fdk_ms": round(ms, 3), "gpu_gups": round(upd / (ms * 1e-3) / 1e9, 2)},
+            out, raw.cpu().numpy())
+
+
+def precision_gpu(p, ell, bp, cos, H, steps: int = 8):
+    """FP16 / BF16 projection storage at the 2048^3 geometry.
+
+    Throughput: `steps` 32-view passes over the whole 2048^3 volume from views
+    stored in each 16-bit type (scaled per view to [0, 1]), timed like the FP32
+    line.  Accuracy: the central 256^3 sub-volume reconstructed from every 64th
+    of the 4096 views with FP32, FP16 and BF16 storage (returned with the raw
+    views, for the FP64 comparison of the CPU leg)."""
+    import numpy as np
+    import torch
+    import fdk_amd as F
+    B = F.MAX_PROJ
+    n = p.nx
+    nv = steps * B
+    raw = torch.empty((B, p.nv, p.nu), device="cuda")
+    mats = p.proj_matrices(0, nv).astype(np.float32)
+    st = torch.cuda.current_stream()
+    timing = {}
+    for name, dt in (("f16", F.FDK_F16), ("bf16", F.FDK_BF16)):
+        q = torch.empty((nv, p.nu, p.nv), dtype=torch.int16, device="cuda")
+        so = np.zeros((nv, 2), np.float32)
+        for b in range(0, nv, B):
+            F.phantom_project(ell, p, b, B, raw)
+            so[b:b + B] = F.filter_views(raw, q, cos, H, F.PAD_MEAN4, True, dtype=dt,
+                                         dst_view_offset=b)
+        bp.zero()
+        bp.bp_device(q, p.nv, p.nu, mats[:B], B, dtype=dt, scale_offset=so[:B])  # warm-up
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        t0 = time.perf_counter()
+        evs[0].record(st)
+        for i in range(steps):
+            bp.bp_device(q, p.nv, p.nu, mats[i * B:(i + 1) * B], B, dtype=dt, offset_views=i * B,
+                         scale_offset=so[i * B:(i + 1) * B])
+            evs[i + 1].record(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+        upd = float(n) ** 3 * B * steps
+        timing[name] = {"gups": round(upd / wall / 1e9, 2), "passes": steps,
+                        "ms_per_pass": round(float(np.mean(ms)), 3),
+                        "bytes_per_update": 4 * 2 + 8.0 / B}
+        del q
+    # central 256^3 from every 64th view, FP32 / FP16 / BF16 storage
+    sub, V = 256, 64
+    ps = F.Parameter(nx=sub, ny=sub, nz=sub, dx=p.dx, dy=p.dy, dz=p.dz, nu=p.nu, nv=p.nv,
+                     du=p.du, dv=p.dv, sod=p.sod, sid=p.sid, nproj=V, direction=p.direction,
+                     scan_angle_deg=p.scan_angle_deg, volume_position_angle_rad=0.0,
+                     filter_type=5)
+    rawv = torch.empty((V, p.nv, p.nu), device="cuda")
+    F.phantom_project(ell, ps, 0, V, rawv)
+    mats_s = ps.proj_matrices().astype(np.float32)
+    vols = {}
+    half = sub // 2
+    for name, dt in (("f32", F.FDK_F32), ("f16", F.FDK_F16), ("bf16", F.FDK_BF16)):
+        q = torch.empty((V, p.nu, p.nv), device="cuda",
+                        dtype=torch.float32 if dt == F.FDK_F32 else torch.int16)
+        so = F.filter_views(rawv, q, cos, H, F.PAD_MEAN4, True, dtype=dt)
+        b2 = F.BackProjectionFDK(p.nv, p.nu, B, sub, sub, sub, (0, 0, 0, sub, sub, half),
+                                 (0, 0, half, sub, sub, half))
+        b2.set_stream(st.cuda_stream)
+        for b in range(0, V, B):
+            b2.bp_device(q, p.nv, p.nu, mats_s[b:b + B], B, dtype=dt, offset_views=b,
+                         scale_offset=None if so is None else so[b:b + B])
+        b2.sync()
+        vols[name] = b2.GetVolumeData()
+        b2.close()
+        del q
+    return timing, {"geometry": ps, "vols": vols, "raw": rawv.cpu().numpy()}
+
+
+def cpu_leg(p, frames_fn, seconds: float, checks: dict):
+    """The CPU baseline leg: the reference's CPU path timed on this host's
+    cores, and every comparison of GPU results with the oracle.
+
+    * 3dct BackProjection<float> (compiled from the reference sources into
+      oracle/_ref; the oracle port when that is absent) on the central 256^3
+      sub-volume of the 2048^3 geometry, about `seconds` of CPU work, plus the
+      3dct row filter per 2048^2 view; extrapolated to the whole scan.
+    * Config 1 in full: the 3dct CPU FDK (its own cos weighting, FP64 row
+      filter and backprojection) timed beside the GPU FDK, and the GPU volume
+      compared with it (its doubled AngularAndrampFactor undone) and with the
+      oracle FDK of the production pairing (mean-4 pad, strict bilinear, FP64).
+    * Parity of the timed kernel: one 32-view pass of the timed views, about
+      3,000 sampled voxels against the oracle's FP64 backprojection.
+    * Storage precision: central 256^3 from FP32 / FP16 / BF16 views against
+      an FP64 reconstruction (FP64 filter, matrices and accumulation)."""
     import numpy as np
     import oracle as O
+    info = host_info()
     sub = min(256, p.nx)
     g = O.Geom(nx=sub, ny=sub, nz=sub, dx=p.dx, dy=p.dy, dz=p.dz, nu=p.nu, nv=p.nv, du=p.du,
                dv=p.dv, sod=p.sod, sid=p.sid, direction=p.direction, nproj=p.nproj,
                scan_angle_deg=p.scan_angle_deg, volume_position_angle_rad=0.0, filter_type=5)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     use_ref = O.ref_available()
+    out = {}
     probe = 2
-    frames = proj_t_host_fn(probe)
+    frames = frames_fn(probe)
     mats = O.proj_matrices(g)
     t0 = time.perf_counter()
     if use_ref:
@@ -103,7 +270,7 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
     per_view = (time.perf_counter() - t0) / probe
     # bounded sample: about `seconds` of CPU work, at most 128 views (2 GiB of views)
     nv = int(max(probe, min(128, p.nproj, seconds / max(per_view, 1e-6))))
-    frames = proj_t_host_fn(nv)
+    frames = frames_fn(nv)
     t0 = time.perf_counter()
     reps = 0
     while True:  # repeat the bounded call until about `seconds` of CPU work
@@ -115,33 +282,88 @@ def cpu_baseline(p, proj_t_host_fn, seconds: float):
         if time.perf_counter() - t0 >= seconds or reps >= 64:
             break
     dt = time.perf_counter() - t0
+    del frames
     nv_total = nv * reps
     gups = sub ** 3 * nv_total / dt / 1e9
-    extra = {}
     if use_ref:
         # the reference's 3dct row filter (FP64 FFT, filterImageRow) on whole
         # views, one view per host thread, for the extrapolated full-scan time
         from concurrent.futures import ThreadPoolExecutor
         H = p.filter()
         rng = np.random.default_rng(0)
-        views = [rng.standard_normal((p.nv, p.nu)).astype(np.float32) for _ in range(threads)]
+        views = [rng.standard_normal((p.nv, p.nu)).astype(np.float32)
+                 for _ in range(info["cores"])]
         t1 = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
+        with ThreadPoolExecutor(info["cores"]) as ex:
             list(ex.map(lambda v: O.ref_filter_rows_3dct(v, H), views))
-        per_view = (time.perf_counter() - t1) / len(views)
+        fpv = (time.perf_counter() - t1) / len(views)
         bp_s = float(p.nx) * p.ny * p.nz * p.nproj / (gups * 1e9)
-        extra = {"filter_s_per_view": round(per_view, 4),
-                 "extrapolated_full_scan_s": {"bp": round(bp_s, 1),
-                                              "filter": round(per_view * p.nproj, 1),
-                                              "note": "linear extrapolation in voxel updates and "
-                                                      "views from the measured samples"}}
-    return {"value": round(gups, 4), "unit": "GUPS", "cores": threads, **extra,
-            "kind": "reference" if use_ref else "port",
-            "sample": (f"central {sub}^3 sub-volume of the {p.nx}^3 geometry x {nv} views "
-                       f"x {reps} repeats ({sub ** 3 * nv_total:.3g} voxel-updates, {dt:.1f} s), "
-                       "prefiltered FP32 "
-                       "views, " + ("3dct BackProjection<float> compiled from the reference "
-                                    "sources" if use_ref else "oracle port (FP64)"))}
+        out["filter_s_per_view"] = round(fpv, 4)
+        out["extrapolated_full_scan_s"] = {"bp": round(bp_s, 1), "filter": round(fpv * p.nproj, 1),
+                                           "note": "linear extrapolation in voxel updates and "
+                                                   "views from the measured samples"}
+    out = {"value": round(gups, 4), "unit": "GUPS", **info,
+           "kind": "reference" if use_ref else "port",
+           "sample": (f"central {sub}^3 sub-volume of the {p.nx}^3 geometry x {nv} views "
+                      f"x {reps} repeats ({sub ** 3 * nv_total:.3g} voxel-updates, {dt:.1f} s), "
+                      "prefiltered FP32 views, " + ("3dct BackProjection<float> compiled from "
+                                                    "the reference sources" if use_ref else
+                                                    "oracle port (FP64)")),
+           **out}
+
+    # ---- config 1 in full
+    c1 = checks.get("c1")
+    if c1 is not None:
+        p1 = c1_geometry()
+        g1 = O.Geom(**{k: getattr(p1, k) for k in O.Geom.__dataclass_fields__})
+        raw, vol = c1["raw"], c1["vol"]
+        r = dict(c1["timing"])
+        if use_ref:
+            t0 = time.perf_counter()
+            ref = O.ref_bp_3dct(g1, raw, prefiltered=False)
+            r["cpu_3dct_fdk_s"] = round(time.perf_counter() - t0, 3)
+            r["cpu_3dct_gups"] = round(256.0 ** 3 * 360 / r["cpu_3dct_fdk_s"] / 1e9, 3)
+            r["speedup_gpu_vs_cpu"] = round(r["cpu_3dct_fdk_s"] / (r["gpu_fdk_ms"] * 1e-3), 1)
+            af = O.angular_ramp_factor(g1)
+            r["rel_rmse_vs_3dct"] = rel_rmse(vol, ref / af)
+            r["rel_rmse_vs_3dct_note"] = ("3dct's CPU FDK (3-sample mean pad, truncating "
+                                          "bilinear) with its doubled AngularAndrampFactor "
+                                          "divided out; the difference is the two conventions")
+        Hf = p1.filter().astype(np.float32).astype(np.float64)
+        filt = O.filter_rows(O.cos_weight(raw, p1.cos_table()), Hf, O.PAD_MEAN4)
+        ref = O.bp(g1, filt, p1.proj_matrices().astype(np.float32).astype(np.float64),
+                   O.BP_GPU_STRICT)
+        r["rel_rmse_vs_oracle"] = rel_rmse(vol, ref)
+        r["oracle_pairing"] = "mean-4 pad, FP32 filter, strict bilinear, FP64 accumulation"
+        out["c1_full"] = r
+
+    # ---- parity of the timed kernel
+    par = checks.get("parity")
+    if par is not None:
+        pp = par["geometry"]
+        gp = O.Geom(**{k: getattr(pp, k) for k in O.Geom.__dataclass_fields__})
+        ref = O.bp_voxels(gp, par["frames"], par["mats"].astype(np.float64), par["ijk"],
+                          O.BP_GPU_STRICT)
+        checks["parity_result"] = {
+            "rel_rmse": rel_rmse(par["got"], ref),
+            "max_abs_over_max": float(np.abs(par["got"] - ref).max() / np.abs(ref).max()),
+            "voxels": int(len(ref)), "views": par["views"],
+            "against": "oracle FP64 strict-bilinear backprojection of the same filtered views "
+                       "with the same FP32 matrices"}
+
+    # ---- storage precision against FP64
+    pr = checks.get("precision")
+    if pr is not None:
+        ps = pr["geometry"]
+        gs = O.Geom(**{k: getattr(ps, k) for k in O.Geom.__dataclass_fields__})
+        f64 = O.filter_rows(O.cos_weight(pr["raw"], ps.cos_table()), ps.filter(), O.PAD_MEAN4)
+        ref = O.bp(gs, f64, ps.proj_matrices(), O.BP_GPU_STRICT)
+        mx = float(np.abs(ref).max())
+        checks["precision_result"] = {
+            name: {"rel_rmse": rel_rmse(v, ref), "max_abs": float(np.abs(v - ref).max()),
+                   "max_abs_over_max": float(np.abs(v - ref).max() / mx)}
+            for name, v in pr["vols"].items()}
+    return out
 
 
 def fp_stage_256cube():
@@ -222,22 +444,29 @@ def learned_sparse_view():
     return r
 
 
-def stream_main(args):
-    """Streamed slab: one GPU's share of a z-slab split of a 4096^3 volume (the
-    central mirror pair, 512 slices), every view copied from host memory during
-    the timed region; GUPS counts the slab's voxel updates over wall time."""
+def stream_leg(views: int, pool: int, source: str, warmup: int, log_steps: bool = False):
+    """The streamed slab: one GPU's share of a z-slab split of a 4096^3 volume
+    (the central mirror pair, 512 slices) from 4096^2-pixel views of an
+    8192-view scan, every view copied from page-locked host memory in 32-view
+    batches during the timed region.  source "raw": FP32 raw views, filtered on
+    the GPU on a stream of its own and stored as FP16 (per-view min/max kept on
+    the device); "filtered": FP16 views filtered beforehand.  The host pool of
+    distinct views is cycled through the scan's view indices (timing only: the
+    parity of this path at this geometry is a test).  GUPS counts the slab's
+    voxel updates over wall time."""
     import numpy as np
     import torch
     import fdk_amd as F
 
-    n, V, nzs = 4096, args.stream_views, 512
-    p = geometry(n, max(V, 32))
+    n, nzs, V = 4096, 512, 8192
+    views = max(F.MAX_PROJ, views // F.MAX_PROJ * F.MAX_PROJ)
+    p = geometry(n, V)
     dev = torch.device("cuda:0")
     B = F.MAX_PROJ
-    pool = max(B, min(args.stream_pool, V) // B * B)
+    pool = max(B, min(pool, views) // B * B)
     store_dt = F.FDK_F16
-    raw_mode = args.stream_source == "raw"
-    log(f"stream: {n}^2 x {nzs} slab pair of {n}^3, {V} views of {n}^2, pool {pool} "
+    raw_mode = source == "raw"
+    log(f"stream: {n}^2 x {nzs} slab pair of {n}^3, {views} of {V} views of {n}^2, pool {pool} "
         f"{'raw FP32' if raw_mode else 'filtered FP16'} views in page-locked memory")
     ell = F.random_ellipsoids(80, seed=3, radius_mm=0.45 * n * p.dx, half_height_mm=0.45 * n * p.dz,
                               axes_mm=(5.0 * p.dx / 0.1333, 100.0 * p.dx / 0.1333))
@@ -272,7 +501,7 @@ def stream_main(args):
         else:
             pipe.push_filtered(host[s0:s0 + B], B, v0, 1, so_pool[s0:s0 + B])
 
-    for w in range(max(args.warmup, 1)):  # warm-up batches, backprojected and then cleared
+    for w in range(max(warmup, 1)):  # warm-up batches, backprojected and then cleared
         push(w * B % V)
     pipe.finish()
     pipe.bp.zero()
@@ -280,12 +509,14 @@ def stream_main(args):
     warm = pipe.stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for v0 in range(0, V - V % B, B):
-        push(v0)
+    for i, v0 in enumerate(range(0, views, B)):
+        push(v0 % V)
+        if log_steps and i % 32 == 31:
+            log(f"stream: {i + 1} batches pushed")
     pipe.finish()
     wall = time.perf_counter() - t0
     st = pipe.stats()
-    views = st["views"] - warm["views"]
+    done = st["views"] - warm["views"]
     bp_ms = st["bp_ms"] - warm["bp_ms"]
     filter_ms = st["filter_ms"] - warm["filter_ms"]
     h2d = st["h2d_bytes"] - warm["h2d_bytes"]
@@ -293,43 +524,101 @@ def stream_main(args):
     probe = vol[n // 2, n // 2, ::31].float().cpu().numpy()
     pipe.close()
     if not np.all(np.isfinite(probe)):
-        log("non-finite voxels in the result")
-        sys.exit(1)
-    upd = float(n) * n * nzs * views
+        raise RuntimeError("non-finite voxels in the streamed slab")
+    upd = float(n) * n * nzs * done
+    return {"workload": f"streamed FDK, {n}x{n}x{nzs} mirror slab pair of a {n}^3 volume "
+                        f"from {done} of the {V} views of {n}x{n} pixels (0.1 mm), 32-view "
+                        "batches copied host-to-device during the timed region",
+            "source": source, "storage": "f16", "views": done, "batches": done // B,
+            "gups": round(upd / wall / 1e9, 2), "wall_s": round(wall, 3),
+            "ms_per_batch": round(wall / (done // B) * 1e3, 3),
+            "bp_kernel_s": round(bp_ms / 1e3, 3), "filter_s": round(filter_ms / 1e3, 3),
+            "filter_stream": "own stream, joined to the backprojection by events"
+                             if raw_mode else None,
+            "h2d_GB": round(h2d / 1e9, 2), "h2d_GBps": round(h2d / wall / 1e9, 1),
+            "bp_busy_frac": round(bp_ms / 1e3 / wall, 4),
+            "pool_views": pool, "voxel_mm": round(p.dx, 5), "pixel_mm": p.du,
+            "slab": [top, bottom]}
+
+
+def stream_main(args):
+    """--stream: the whole streamed 8192-view scan as its own JSON line."""
+    r = stream_leg(args.stream_views, args.stream_pool, args.stream_source, args.warmup,
+                   log_steps=True)
     out = {
         "metric": "streamed FDK backprojection GUPS, 4096^2 x 512 slab from 8192 views",
-        "value": round(upd / wall / 1e9, 2), "unit": "GUPS", "n_gpus": 1,
-        "steps": views // B, "warmup": max(args.warmup, 1),
-        "ms_per_step": round(wall / (views // B) * 1e3, 3), "higher_is_better": True,
+        "value": r["gups"], "unit": "GUPS", "n_gpus": 1,
+        "steps": r["batches"], "warmup": max(args.warmup, 1),
+        "ms_per_step": r["ms_per_batch"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f16-storage/f32",
         "data": "synthetic (exact line integrals of a seeded 80-ellipsoid phantom; "
-                f"{pool} distinct views cycled from page-locked host memory)",
-        "config": {"workload": f"streamed FDK, {n}x{n}x{nzs} mirror slab pair of a {n}^3 volume "
-                               f"from {views} views of {n}x{n} pixels, 32-view batches copied "
-                               "host-to-device during the timed region",
-                   "source": args.stream_source, "voxel_mm": round(p.dx, 5), "pixel_mm": p.du,
-                   "slab": [top, bottom]},
-        "pipeline": {"wall_s": round(wall, 3), "bp_kernel_s": round(bp_ms / 1e3, 3),
-                     "filter_s": round(filter_ms / 1e3, 3),
-                     "h2d_GB": round(h2d / 1e9, 2), "h2d_GBps_avg": round(h2d / wall / 1e9, 1),
-                     "bp_busy_frac": round(bp_ms / 1e3 / wall, 4)},
+                f"{r['pool_views']} distinct views cycled from page-locked host memory)",
+        "config": {"workload": r["workload"], "source": r["source"], "voxel_mm": r["voxel_mm"],
+                   "pixel_mm": r["pixel_mm"], "slab": r["slab"]},
+        "pipeline": r,
     }
     print(json.dumps(out), flush=True)
+
+
+def load_counters(kernel: str, dt: str, n: int):
+    """The committed counter summary of the backprojection kernel at this
+    workload (profiles/<kernel>_<dtype>_<n>cube_counters.json): per-voxel-update
+    instruction counts and HBM traffic per launch, read from rocprofv3 runs of
+    this same bench (the summary names its commands)."""
+    path = os.path.join(HERE, "profiles", f"{kernel}_{dt}_{n}cube_counters.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        return json.load(fh), "profiles/" + os.path.basename(path)
+
+
+def roofline_of(kernel: str, dt: str, n: int, avg_ms: float, upd_per_launch: float,
+                b_upd: float):
+    """Roofline of the dominant kernel.  The bound is the resource the committed
+    counters show closest to its peak (VALU issue for the LDS-staged kernel);
+    `achieved` is that resource's per-update figure from the counter summary
+    times this launch's updates over the live event-timed launch duration.  The
+    algorithmic HBM figure (4 taps x sample bytes + 8 B / 32 views per update,
+    all taps counted as if gathered from HBM) is kept as `algorithmic_*`: the
+    kernel serves the taps from LDS-staged strips, so that figure exceeds the
+    HBM peak and bounds nothing."""
+    algo = b_upd * upd_per_launch / (avg_ms * 1e-3)
+    r = {"kernel": kernel, "kernel_ms_avg": round(avg_ms, 3),
+         "algorithmic_bytes_per_update": b_upd,
+         "algorithmic_GBps": round(algo / 1e9, 1),
+         "algorithmic_frac": round(algo / HBM_PEAK, 4)}
+    c, src = load_counters(kernel, dt, n)
+    if c is None:
+        r.update({"bound": "hbm", "achieved": round(algo / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                  "unit": "GB/s", "frac": None, "traffic": None,
+                  "note": "no counter summary for this kernel and workload"})
+        return r
+    valu = c["valu_insts_per_update"] * upd_per_launch / (avg_ms * 1e-3)
+    hbm = c["hbm_bytes_per_launch"] / (avg_ms * 1e-3)
+    l2 = c.get("l2_bytes_per_launch")
+    r.update({
+        "bound": "valu",
+        "achieved": round(valu / 1e9, 1),
+        "peak": round(VALU_PEAK / 1e9, 1),
+        "unit": "G VALU wave-instructions/s",
+        "frac": round(valu / VALU_PEAK, 4),
+        "traffic": c["hbm_bytes_per_launch"],
+        "traffic_source": src,
+        "valu_insts_per_update": c["valu_insts_per_update"],
+        "lds_insts_per_update": c.get("lds_insts_per_update"),
+        "hbm_GBps": round(hbm / 1e9, 1),
+        "hbm_frac": round(hbm / HBM_PEAK, 4),
+        "l2_GBps": round(l2 / (avg_ms * 1e-3) / 1e9, 1) if l2 else None,
+        "counters": src,
+    })
+    return r
 
 
 def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
-        return
-    if args.stream and args.gpus == 1:
-        import torch
-        if not torch.cuda.is_available():
-            log("no GPU visible")
-            sys.exit(1)
-        sys.path.insert(0, HERE)
-        stream_main(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path runs "
@@ -344,12 +633,14 @@ def main():
     if not torch.cuda.is_available():
         log("no GPU visible")
         sys.exit(1)
+    if args.stream:
+        stream_main(args)
+        return
     dev = torch.device("cuda:0")
     n, V = args.n, args.views
     p = geometry(n, V)
     B = F.MAX_PROJ
     dtype = {"f32": F.FDK_F32, "f16": F.FDK_F16, "bf16": F.FDK_BF16}[args.dtype]
-    need = (args.warmup + args.steps) * B
     resident = V if args.resident_views <= 0 else min(V, max(args.resident_views, B))
     resident = min(resident, V)
     log(f"geometry {n}^3 voxels of {p.dx:.4f} mm, {n}^2 detector of {p.du:.3f} mm, {V} views; "
@@ -438,50 +729,85 @@ def main():
     if not np.all(np.isfinite(probe)):
         log("non-finite voxels in the result")
         sys.exit(1)
+    del vol
 
     upd = float(n) ** 3 * views_done
     gups = upd / wall / 1e9
     avg_ms = float(np.mean(kern_ms))
     s_bytes = 4 if dtype == F.FDK_F32 else 2
     b_upd = 4 * s_bytes + 8.0 / B          # 4 taps + volume read/write per 32-view pass
-    algo_bytes = b_upd * float(n) ** 3 * B  # per launch (one 32-view pass)
-    achieved = algo_bytes / (avg_ms * 1e-3)
     # the library's choice for this workload: FP32 views use the prefetching
     # LDS kernel, 16-bit views the LDS kernel; FDK_BP_KERNEL=v2 / v3 force the
     # direct-gather / non-prefetching kernels
     forced = os.environ.get("FDK_BP_KERNEL")
     kernel = ("bp_column_kernel" if forced == "v2" else
               "bp_lds_kernel" if forced == "v3" or dtype != F.FDK_F32 else "bp_lds_pf_kernel")
-    traffic = None
-    tpath = os.path.join(HERE, "profiles", f"{kernel}_{args.dtype}_{n}cube_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = roofline_of(kernel, args.dtype, n, avg_ms, float(n) ** 3 * B, b_upd)
+
+    checks = {}
+    extra_legs = not args.no_cpu_baseline and dtype == F.FDK_F32
+    precision = None
+    if extra_legs:
+        # parity of the timed kernel: the first timed pass again, alone, into a
+        # cleared volume; sampled voxels go to the CPU leg
+        rng = np.random.default_rng(11)
+        ijk = rng.integers(0, n, (3000, 3)).astype(np.int32)
+        bp.zero()
+        cnt = step(args.warmup)
+        torch.cuda.synchronize()
+        vol = bp.volume_tensor()  # [x][y][zz]; the slab pair covers z in order, so zz = z
+        idx = [torch.as_tensor(ijk[:, a].astype(np.int64), device=dev) for a in range(3)]
+        got = vol[idx[0], idx[1], idx[2]].double().cpu().numpy()
+        del vol
+        v0 = (args.warmup * B) % V
+        s0 = v0 % resident
+        checks["parity"] = {"geometry": p, "ijk": ijk, "got": got, "views": [v0, v0 + cnt],
+                            "frames": np.ascontiguousarray(
+                                store[s0:s0 + cnt].transpose(1, 2).cpu().numpy()),
+                            "mats": mats[v0:v0 + cnt]}
+        log("precision leg (FP16 / BF16 storage)")
+        ptiming, checks["precision"] = precision_gpu(p, ell, bp, cos, H)
+        precision = {"timing": ptiming}
+        log(f"precision timing: {ptiming}")
+        log("config 1 on the GPU")
+        c1_timing, c1_vol, c1_raw = c1_gpu()
+        checks["c1"] = {"timing": c1_timing, "vol": c1_vol, "raw": c1_raw}
 
     cpu = None
     if not args.no_cpu_baseline:
-        sub = min(256, n)
-
         def frames(k):
             # the same filtered views, untransposed, for the CPU sub-volume
             nk = min(k, resident)
-            x = store[:nk]
-            if dtype != F.FDK_F32:
-                return None
-            return np.ascontiguousarray(x.transpose(1, 2).cpu().numpy())
-        log("timing the reference CPU backprojection")
+            return np.ascontiguousarray(store[:nk].transpose(1, 2).cpu().numpy())
+        log("CPU leg: reference CPU backprojection, config 1 in full, oracle checks")
         try:
-            cpu = cpu_baseline(p, frames, args.cpu_seconds) if dtype == F.FDK_F32 else None
+            cpu = cpu_leg(p, frames, args.cpu_seconds, checks) if dtype == F.FDK_F32 else None
         except Exception as e:  # the baseline never decides the GPU result
-            log(f"cpu baseline failed: {e}")
-            cpu = None
+            log(f"cpu leg failed: {e!r}")
+            cpu = {"error": repr(e)}
+    parity = checks.get("parity_result")
+    if precision is not None:
+        precision["vs_fp64_central_256cube"] = checks.get("precision_result")
+        precision["fp64_reference"] = ("central 256^3 of the 2048^3 geometry from every 64th of "
+                                       "the 4096 views: oracle FP64 filter (FP64 H), FP64 "
+                                       "matrices, FP64 strict-bilinear accumulation")
+    bp.close()
+    del store, raw, checks
+    torch.cuda.empty_cache()
+
+    stream_r = None
+    if not args.no_stream:
+        log("streamed slab leg (4096^2 x 512 from 4096^2 views, raw FP32 host views)")
+        try:
+            stream_r = stream_leg(args.stream_leg_views, args.stream_pool, "raw", 1)
+        except Exception as e:  # reported, never decides the backprojection result
+            log(f"stream leg failed: {e!r}")
+            stream_r = {"error": repr(e)}
+        torch.cuda.empty_cache()
 
     learned = None
     if not args.no_learned:
         log("learned sparse-view comparison (1024^3)")
-        bp.close()
-        del store, raw, vol
-        torch.cuda.empty_cache()
         try:
             learned = learned_sparse_view()
         except Exception as e:  # reported, never decides the backprojection result
@@ -506,18 +832,16 @@ def main():
                    "volume": [n, n, n], "detector": [n, n], "views": V, "views_per_step": B,
                    "voxel_mm": round(p.dx, 5), "pixel_mm": p.du, "sod_mm": p.sod, "sdd_mm": p.sid,
                    "projection_storage": args.dtype, "resident_views": resident},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": traffic,
-                     "bytes_per_update": b_upd, "kernel": kernel,
-                     "kernel_ms_avg": round(avg_ms, 3)},
+        "roofline": roofline,
+        "parity": parity,
         "cpu_baseline": cpu,
+        "precision": precision,
+        "stream": stream_r,
         "filter_stage": filter_stage,
         "fp_stage": fp_stage,
         "learned": learned,
     }
     print(json.dumps(out), flush=True)
-    bp.close()
 
 
 if __name__ == "__main__":

@@ -30,6 +30,7 @@ FDK_F32, FDK_F16, FDK_BF16 = 0, 1, 2
 PAD_ZERO, PAD_MEAN3, PAD_MEAN4 = 0, 3, 4
 FP_CLAMP, FP_ZERO = 0, 1
 MAX_PROJ = 32
+FDK_ERR_INVALID, FDK_ERR_OOM, FDK_ERR_HIP, FDK_ERR_UNSUPPORTED = -1, -2, -3, -4
 FILTER_TYPES = ["none", "ram-lak", "shepp-logan", "cosine", "hamming", "hann"]
 
 
@@ -75,6 +76,8 @@ _SIGS = {
                                C.c_int]),
     "fdk_bp_batch_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                       C.c_int64, C.c_void_p, C.c_void_p, C.c_int]),
+    "fdk_bp_batch_device_minmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                             C.c_int64, C.c_void_p, C.c_void_p, C.c_int]),
     "fdk_bp_sync": (C.c_int, [C.c_void_p]),
     "fdk_bp_count": (C.c_int, [C.c_void_p]),
     "fdk_bp_kernel_ms": (C.c_float, [C.c_void_p]),
@@ -97,6 +100,9 @@ _SIGS = {
     "fdk_filter_views": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64] + [C.c_int] * 4
                          + [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                             C.c_void_p]),
+    "fdk_filter_views_async": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+                               + [C.c_int] * 4 + [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fdk_filter_proj_image": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "fdk_dev_malloc": (C.c_int, [C.c_void_p, C.c_size_t]),

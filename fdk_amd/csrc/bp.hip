@@ -70,7 +70,25 @@ struct BpArgs {
   int view_bytes32;      // W * H * sample bytes (< 2^31), the range of a view's resource
   int row_bytes;         // W * sample bytes
   int buf16;             // 16-bit samples with 4-byte aligned rows and views (buffer path)
+  const unsigned* mm;    // per-view {min, max} words on the device (16-bit views), or null
 };
+
+// The filter's per-view {min, max} of a 16-bit batch, stored as order-preserving
+// uints (negative floats bit-inverted, others with the sign bit set).
+__device__ __forceinline__ float ord_to_float(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// Dequantisation of view s, p = q * sc + of: from the kernel arguments, or,
+// when the batch's min/max stayed on the device, sc = max - min and of = min
+// (the same float arithmetic the host applies to the words it reads back).
+__device__ __forceinline__ float2 view_so(const BpArgs& a, const BpScale& S, int s) {
+  if (a.mm) {
+    const float lo = ord_to_float(a.mm[2 * s]), hi = ord_to_float(a.mm[2 * s + 1]);
+    return make_float2(hi - lo, lo);
+  }
+  return make_float2(S.so[s][0], S.so[s][1]);
+}
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
@@ -230,9 +248,10 @@ __global__ __launch_bounds__(256) void bp_column_kernel(BpArgs a, BpMats M, BpSc
     float w1 = uok ? fu * Z2 : 0.0f;
     float off = 0.0f;
     if (DT != FDK_F32) {
-      w0 *= S.so[s][0];
-      w1 *= S.so[s][0];
-      off = S.so[s][1] * Z2;  // bilinear weights sum to 1 inside the view
+      const float2 so = view_so(a, S, s);
+      w0 *= so.x;
+      w1 *= so.x;
+      off = so.y * Z2;  // bilinear weights sum to 1 inside the view
     }
     if constexpr (DT == FDK_F32) {
       // Two buffer resources per view: rows u (from the view start) and u+1
@@ -651,16 +670,16 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a, BpMats M, BpScale
     if (lane == 0) footprint_put<SYM>(info, wave, c, zt0, zt1, zm0, zm1);
     __syncthreads();
     const Strip st = strip_get<SYM>(info, a.W, R, CT);
+    const float2 so = view_so(a, S, s);
     if (st.fits)
-      strip_stage<DT, SYM>(st, view_rsrc(a, s), a.W, CT, S.so[s][0], S.so[s][1], stage, stm, wave,
-                           lane);
+      strip_stage<DT, SYM>(st, view_rsrc(a, s), a.W, CT, so.x, so.y, stage, stm, wave, lane);
     __syncthreads();  // every wave has read info; the next view writes it after its compute
     if (st.fits)
       compute_lds<SYM, ZPT>(acc, accm, zt, zm, c, st, stage, stm, a.W, R, CT);
     else if (c.uok)
       compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, c, a,
                                    static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
-                                   S.so[s][0], S.so[s][1]);
+                                   so.x, so.y);
   }
   if (active) column_write<SYM, ZPT>(a, xs, ys, zbase, acc, accm);
 }
@@ -682,9 +701,10 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
   }
   __syncthreads();
   Strip cur = strip_get<SYM>(info[0], a.W, R, CT);
-  if (cur.fits)
-    strip_stage<DT, SYM>(cur, view_rsrc(a, 0), a.W, CT, S.so[0][0], S.so[0][1], stage, stm, wave,
-                         lane);
+  if (cur.fits) {
+    const float2 so0 = view_so(a, S, 0);
+    strip_stage<DT, SYM>(cur, view_rsrc(a, 0), a.W, CT, so0.x, so0.y, stage, stm, wave, lane);
+  }
   for (int s = 0; s < a.nproj; ++s) {
     const bool more = s + 1 < a.nproj;
     if (more && lane == 0) {
@@ -702,14 +722,17 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
     const ColView cv = col_view(M.m[s], x, y, a.H, active);
     if (cur.fits)
       compute_lds<SYM, ZPT>(acc, accm, zt, zm, cv, cur, stage, stm, a.W, R, CT);
-    else if (cv.uok)
+    else if (cv.uok) {
+      const float2 so = view_so(a, S, s);
       compute_direct<DT, SYM, ZPT>(acc, accm, zt, zm, cv, a,
                                    static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
-                                   S.so[s][0], S.so[s][1]);
+                                   so.x, so.y);
+    }
     __syncthreads();  // every wave is done with strip s
-    if (nxt.fits)
-      strip_commit<DT, SYM>(pre, nxt, a.W, CT, S.so[s + 1][0], S.so[s + 1][1], stage, stm, wave,
-                            lane);
+    if (nxt.fits) {
+      const float2 so = view_so(a, S, s + 1);
+      strip_commit<DT, SYM>(pre, nxt, a.W, CT, so.x, so.y, stage, stm, wave, lane);
+    }
     cur = nxt;
   }
   if (active) column_write<SYM, ZPT>(a, xs, ys, zbase, acc, accm);
@@ -746,12 +769,13 @@ __global__ __launch_bounds__(256) void bp_general_kernel(BpArgs a, BpMats M, BpS
       const bool uok = (unsigned)iu <= (unsigned)(a.H - 2);
       const float fu = U - fu0, Z2 = Z * Z;
       float w0 = uok ? (1.f - fu) * Z2 : 0.f, w1 = uok ? fu * Z2 : 0.f;
-      if (DT != FDK_F32) { w0 *= S.so[s][0]; w1 *= S.so[s][0]; }
+      const float2 so = view_so(a, S, s);
+      if (DT != FDK_F32) { w0 *= so.x; w1 *= so.x; }
       const long long rowoff = (long long)s * a.view_stride + (long long)(uok ? iu : 0) * a.W;
       const void* rows = (DT == FDK_F32) ? (const void*)(static_cast<const float*>(a.proj) + rowoff)
                                          : (const void*)(static_cast<const unsigned short*>(a.proj) + rowoff);
       float c = sample_v<DT>(rows, V, a.W, w0, w1);
-      if (DT != FDK_F32) c += (uok && V >= 0.f && V < (float)(a.W - 1)) ? S.so[s][1] * Z2 : 0.f;
+      if (DT != FDK_F32) c += (uok && V >= 0.f && V < (float)(a.W - 1)) ? so.y * Z2 : 0.f;
       acc[q] += c;
     }
   }
@@ -857,6 +881,7 @@ int fill_args(fdk_bp* h, int W, int H, long long view_stride, int esize, int npr
   a.row_bytes = W * esize;
   a.buf16 = esize == 2 && (W % 2 == 0) && (view_stride % 2 == 0) &&
             (reinterpret_cast<uintptr_t>(proj) % 4 == 0);
+  a.mm = nullptr;
   return FDK_OK;
 }
 
@@ -975,7 +1000,8 @@ int launch_bp_dt(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, 
 }
 
 int launch_bp(fdk_bp* h, int dtype, const void* proj, int W, int H, long long view_stride,
-              const float* mats, const float* scale_offset, int nproj) {
+              const float* mats, const float* scale_offset, int nproj,
+              const unsigned* dev_minmax = nullptr) {
   FDK_REQUIRE(nproj >= 0 && nproj <= kMaxProj, "nProj %d exceeds %d views per pass", nproj, kMaxProj);
   FDK_REQUIRE(mats != nullptr || nproj == 0, "matrices are NULL");
   if (nproj == 0) return FDK_OK;
@@ -993,6 +1019,7 @@ int launch_bp(fdk_bp* h, int dtype, const void* proj, int W, int H, long long vi
   }
   BpArgs a;
   if (int r = fill_args(h, W, H, view_stride, dtype == FDK_F32 ? 4 : 2, nproj, proj, a)) return r;
+  a.mm = dtype == FDK_F32 ? nullptr : dev_minmax;
   h->count += nproj;
   switch (dtype) {
     case FDK_F32: return launch_bp_dt<FDK_F32>(h, a, M, S, general);
@@ -1140,6 +1167,21 @@ extern "C" int fdk_bp_batch_device(fdk_bp_t h, int transpose, const void* dev_pr
                                nproj, view_stride, view, h->stream))
     return r;
   return launch_bp(h, FDK_F32, h->staging, W, H, (long long)view, mats, nullptr, nproj);
+}
+
+extern "C" int fdk_bp_batch_device_minmax(fdk_bp_t h, const void* dev_proj, int dtype, int width,
+                                          int height, int64_t view_stride, const float* mats,
+                                          const unsigned* dev_minmax, int nproj) {
+  FDK_REQUIRE(h, "handle is NULL");
+  FDK_REQUIRE(dtype == FDK_F16 || dtype == FDK_BF16, "dtype %d is not a 16-bit storage type", dtype);
+  FDK_REQUIRE(dev_proj != nullptr || nproj == 0, "projections are NULL");
+  FDK_REQUIRE(dev_minmax != nullptr || nproj == 0, "device min/max words are NULL");
+  if (nproj <= 0) return FDK_OK;
+  FDK_REQUIRE(nproj <= kMaxProj, "nProj %d exceeds %d views per pass", nproj, kMaxProj);
+  FDK_REQUIRE(view_stride >= (int64_t)width * height, "view_stride %lld is smaller than a view",
+              (long long)view_stride);
+  return launch_bp(h, dtype, dev_proj, width, height, (long long)view_stride, mats, nullptr, nproj,
+                   dev_minmax);
 }
 
 extern "C" int fdk_bp_sync(fdk_bp_t h) {

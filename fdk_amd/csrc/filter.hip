@@ -49,7 +49,14 @@ struct FilterArgs {
   const float2* tw;       // N/2 twiddles e^{-2 pi i k / N}
   int nu, nrows, N, logN, pad;
   unsigned* minmax;       // per view {min, max} as ordered uints, or null
+  // Output mode: kStoreF32 writes dst as floats; kStoreNone writes nothing
+  // (the min/max pass of a two-pass 16-bit filter); kStoreF16 / kStoreBF16
+  // write (p - min) / (max - min) of view {min, max} words qmm as 16-bit.
+  int qmode;
+  const unsigned* qmm;
 };
+
+constexpr int kStoreF32 = 0, kStoreNone = 1, kStoreF16 = 2, kStoreBF16 = 3;
 
 __device__ inline unsigned brev_n(unsigned k, int logN) { return __brev(k) >> (32 - logN); }
 
@@ -324,6 +331,41 @@ __device__ inline unsigned f2ord(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+__device__ inline float ord2f(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// q = (p - min) * (1 / (max - min)) as FP16 / BF16 bits (round to nearest even).
+template <int Q>
+__device__ inline unsigned short quantize16(float p, float lo, float inv) {
+  const float q = (p - lo) * inv;
+  if (Q == kStoreF16) return __half_as_ushort(__float2half_rn(q));
+  const unsigned b = __float_as_uint(q);
+  return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+// Crop, scale by 1/N and store one output sample pair (rows ra, ra+1 of the
+// view) of the 16-bit modes.
+template <int Q>
+__device__ inline void store16(const FilterArgs& a, int view, int u, int ra, bool has_b, float fa,
+                               float fb, float lo, float inv) {
+  unsigned short* d = static_cast<unsigned short*>(static_cast<void*>(a.dst)) + view * a.dst_view;
+  const unsigned short qa = quantize16<Q>(fa, lo, inv);
+  const unsigned short qb = has_b ? quantize16<Q>(fb, lo, inv) : 0;
+  if (a.transpose) {
+    unsigned short* o = d + (long long)u * a.nrows + ra;
+    if (has_b && (a.nrows & 1) == 0) {
+      *reinterpret_cast<unsigned*>(o) = (unsigned)qa | ((unsigned)qb << 16);  // 4-byte aligned
+    } else {
+      o[0] = qa;
+      if (has_b) o[1] = qb;
+    }
+  } else {
+    d[(long long)ra * a.dst_pitch + u] = qa;
+    if (has_b) d[(long long)(ra + 1) * a.dst_pitch + u] = qb;
+  }
+}
+
 // M = log2 N selects the Stockham plan (9..13); M = 0 runs the in-place
 // bit-reversed radix-2^2 FFT for the other lengths.
 template <int M>
@@ -450,10 +492,21 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
   const float inv = 1.0f / (float)a.N;
   float* d = a.dst + view * a.dst_view;
   float lo = INFINITY, hi = -INFINITY;
+  float qlo = 0.f, qinv = 0.f;
+  if (a.qmode == kStoreF16 || a.qmode == kStoreBF16) {
+    qlo = ord2f(a.qmm[2 * view]);
+    const float range = ord2f(a.qmm[2 * view + 1]) - qlo;
+    qinv = range > 0.f ? 1.0f / range : 0.f;
+  }
   for (int u = threadIdx.x; u < nu; u += blockDim.x) {
     const float2 c = buf[M ? sk_slot<sk_pad<M>()>(off + u) : off + u];
     const float fa = c.x * inv, fb = c.y * inv;
-    if (a.transpose) {
+    if (a.qmode == kStoreNone) {
+    } else if (a.qmode == kStoreF16) {
+      store16<kStoreF16>(a, view, u, ra, has_b, fa, fb, qlo, qinv);
+    } else if (a.qmode == kStoreBF16) {
+      store16<kStoreBF16>(a, view, u, ra, has_b, fa, fb, qlo, qinv);
+    } else if (a.transpose) {
       float* o = d + (long long)u * a.nrows + ra;
       if (has_b && (a.nrows & 1) == 0) {
         *reinterpret_cast<float2*>(o) = make_float2(fa, fb);  // 8-byte aligned: u*nrows + ra even
@@ -479,10 +532,6 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
       atomicMax(&a.minmax[2 * view + 1], f2ord(hi));
     }
   }
-}
-
-__device__ inline float ord2f(unsigned o) {
-  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
 // FP32 staging -> FP16/BF16 with q = (p - min) / (max - min) per view.
@@ -525,8 +574,22 @@ __global__ void correct_kernel(const float* __restrict__ src, const float* __res
   dst[o] = src[o] * table[(long long)y * tpitch + x];
 }
 
-// Device twiddle tables, one per (device, N), built in FP64 and kept for the
-// life of the process.
+// Twiddles e^{-2 pi i k / N}, k < N (the Stockham passes index the whole
+// circle), evaluated in FP64 and rounded to float.
+__global__ void twiddle_kernel(float2* tw, int N) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) {
+    const double a = -2.0 * 3.141592653589793238462 * k / N;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+}
+
+void launch_twiddles(float2* tw, int N, hipStream_t st) {
+  hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, st, tw, N);
+}
+
+// Device twiddle tables, one per (device, N), built on the first use and kept
+// for the life of the process.
 std::mutex g_tw_mutex;
 std::map<std::pair<int, int>, float2*> g_tw;
 
@@ -539,14 +602,18 @@ int twiddles(int N, const float2** out) {
     *out = it->second;
     return FDK_OK;
   }
-  std::vector<float2> h(N);  // the Stockham passes index the whole circle
-  for (int k = 0; k < N; ++k) {
-    const double a = -2.0 * 3.141592653589793238462 * k / N;
-    h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-  }
   float2* d = nullptr;
-  FDK_HIP(hipMalloc(&d, sizeof(float2) * h.size()));
-  FDK_HIP(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+  FDK_HIP(hipMalloc(&d, sizeof(float2) * N));
+  hipStream_t st = nullptr;
+  FDK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  launch_twiddles(d, N, st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return fdk::fail(FDK_ERR_HIP, "twiddle table: %s", hipGetErrorString(e));
+  }
   g_tw[{dev, N}] = d;
   *out = d;
   return FDK_OK;
@@ -583,7 +650,8 @@ int launch_filter(FilterArgs& a, int nviews, hipStream_t st) {
   FDK_REQUIRE(a.nrows >= 1 && nviews >= 0, "bad row / view count");
   if (nviews == 0) return FDK_OK;
   a.logN = log2_exact(a.N);
-  if (int r = twiddles(a.N, &a.tw)) return r;
+  if (!a.tw)
+    if (int r = twiddles(a.N, &a.tw)) return r;
   const int plan = (fft_choice() && a.logN >= 9 && a.logN <= 13) ? a.logN : 0;
   const size_t lds = sizeof(float2) * (plan >= 9 && plan <= 12 ? a.N + a.N / 16 : a.N);
   dim3 grid((a.nrows + 1) / 2, nviews);
@@ -639,16 +707,11 @@ extern "C" int fdk_filter_rows(const float* dev_src, float* dev_dst, int width, 
   return launch_filter(a, depth, static_cast<hipStream_t>(stream));
 }
 
-extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst,
-                                int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
-                                const float* dev_cos, const float* dev_filter, int N, int pad_mode,
-                                int transpose, float* host_scale_offset, void* stream) {
-  FDK_REQUIRE(dev_src && dev_dst && dev_filter, "NULL pointer");
-  FDK_REQUIRE(transpose == 0 || transpose == 1, "transpose must be 0 or 1");
-  FDK_REQUIRE(dtype == FDK_F32 || dtype == FDK_F16 || dtype == FDK_BF16, "unknown dtype %d", dtype);
-  FDK_REQUIRE(view_stride_in >= (int64_t)nu * nv && view_stride_out >= (int64_t)nu * nv,
-              "view strides smaller than a view");
-  hipStream_t st = static_cast<hipStream_t>(stream);
+namespace {
+
+FilterArgs view_args(const float* dev_src, int64_t view_stride_in, int nu, int nv,
+                     const float* dev_cos, const float* dev_filter, int N, int pad_mode,
+                     int transpose) {
   FilterArgs a{};
   a.src = dev_src;
   a.src_view = view_stride_in;
@@ -661,6 +724,53 @@ extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, vo
   a.nrows = nv;
   a.N = N;
   a.pad = pad_mode;
+  return a;
+}
+
+// 16-bit output: FP32 filtered batch into `stage`, per-view {min, max} into
+// `mm`, then (p - min) / (max - min) quantized into dst.  All on `st`.
+int filter_views_q(FilterArgs a, int nviews, void* dev_dst, int64_t view_stride_out, int dtype,
+                   float* stage, unsigned* mm, hipStream_t st) {
+  const size_t view = (size_t)a.nu * a.nrows;
+  hipLaunchKernelGGL(minmax_init_kernel, dim3((nviews + 255) / 256), dim3(256), 0, st, mm, nviews);
+  a.dst = stage;
+  a.dst_view = (long long)view;
+  a.minmax = mm;
+  if (int r = launch_filter(a, nviews, st)) return r;
+  const int blocks = (int)std::min<size_t>(1024, (view + 255) / 256);
+  if (dtype == FDK_F16)
+    hipLaunchKernelGGL(quantize_kernel<FDK_F16>, dim3(blocks, nviews), dim3(256), 0, st, stage,
+                       (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
+  else
+    hipLaunchKernelGGL(quantize_kernel<FDK_BF16>, dim3(blocks, nviews), dim3(256), 0, st, stage,
+                       (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+int check_views(const float* dev_src, void* dev_dst, const float* dev_filter, int transpose,
+                int dtype, int nu, int nv, int64_t view_stride_in, int64_t view_stride_out) {
+  FDK_REQUIRE(dev_src && dev_dst && dev_filter, "NULL pointer");
+  FDK_REQUIRE(transpose == 0 || transpose == 1, "transpose must be 0 or 1");
+  FDK_REQUIRE(dtype == FDK_F32 || dtype == FDK_F16 || dtype == FDK_BF16, "unknown dtype %d", dtype);
+  FDK_REQUIRE(nu > 0 && nv > 0, "bad view size %d x %d", nu, nv);
+  FDK_REQUIRE(view_stride_in >= (int64_t)nu * nv && view_stride_out >= (int64_t)nu * nv,
+              "view strides smaller than a view");
+  return FDK_OK;
+}
+
+}  // namespace
+
+extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst,
+                                int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
+                                const float* dev_cos, const float* dev_filter, int N, int pad_mode,
+                                int transpose, float* host_scale_offset, void* stream) {
+  if (int r = check_views(dev_src, dev_dst, dev_filter, transpose, dtype, nu, nv, view_stride_in,
+                          view_stride_out))
+    return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  FilterArgs a = view_args(dev_src, view_stride_in, nu, nv, dev_cos, dev_filter, N, pad_mode,
+                           transpose);
   if (dtype == FDK_F32) {
     a.dst = static_cast<float*>(dev_dst);
     a.dst_view = view_stride_out;
@@ -675,22 +785,7 @@ extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, vo
   unsigned* mm = nullptr;
   FDK_HIP(hipMallocAsync((void**)&tmp, view * nviews * sizeof(float), st));
   FDK_HIP(hipMallocAsync((void**)&mm, 2 * sizeof(unsigned) * nviews, st));
-  hipLaunchKernelGGL(minmax_init_kernel, dim3((nviews + 255) / 256), dim3(256), 0, st, mm, nviews);
-  a.dst = tmp;
-  a.dst_view = (long long)view;
-  a.minmax = mm;
-  int r = launch_filter(a, nviews, st);
-  if (r == FDK_OK) {
-    const int blocks = (int)std::min<size_t>(1024, (view + 255) / 256);
-    if (dtype == FDK_F16)
-      hipLaunchKernelGGL(quantize_kernel<FDK_F16>, dim3(blocks, nviews), dim3(256), 0, st, tmp,
-                         (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
-    else
-      hipLaunchKernelGGL(quantize_kernel<FDK_BF16>, dim3(blocks, nviews), dim3(256), 0, st, tmp,
-                         (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) r = fdk::fail(FDK_ERR_HIP, "quantize: %s", hipGetErrorString(e));
-  }
+  int r = filter_views_q(a, nviews, dev_dst, view_stride_out, dtype, tmp, mm, st);
   std::vector<unsigned> hmm(2 * nviews);
   if (r == FDK_OK) {
     hipError_t e = hipMemcpyAsync(hmm.data(), mm, hmm.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st);
@@ -707,6 +802,23 @@ extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, vo
     host_scale_offset[2 * v + 1] = lo;
   }
   return FDK_OK;
+}
+
+extern "C" int fdk_filter_views_async(const float* dev_src, int64_t view_stride_in, void* dev_dst,
+                                      int64_t view_stride_out, int dtype, int nu, int nv,
+                                      int nviews, const float* dev_cos, const float* dev_filter,
+                                      int N, int pad_mode, int transpose, float* dev_workspace,
+                                      unsigned* dev_minmax, void* stream) {
+  if (int r = check_views(dev_src, dev_dst, dev_filter, transpose, dtype, nu, nv, view_stride_in,
+                          view_stride_out))
+    return r;
+  FDK_REQUIRE(dtype == FDK_F16 || dtype == FDK_BF16, "dtype %d is not a 16-bit storage type", dtype);
+  FDK_REQUIRE(dev_workspace && dev_minmax, "workspace or min/max buffer is NULL");
+  if (nviews <= 0) return FDK_OK;
+  FilterArgs a = view_args(dev_src, view_stride_in, nu, nv, dev_cos, dev_filter, N, pad_mode,
+                           transpose);
+  return filter_views_q(a, nviews, dev_dst, view_stride_out, dtype, dev_workspace, dev_minmax,
+                        static_cast<hipStream_t>(stream));
 }
 
 extern "C" int fdk_filter_proj_image(float* host_proj, int nu, int nv, int nviews,

@@ -1600,9 +1600,10 @@ extern "C" int fdk_nn_box_copy(const float* src, const int src_dims[3], const in
 
 extern "C" int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t seed,
                                  void* stream) {
-  FDK_REQUIRE(dev_proj && i0 > 0 && mu > 0, "bad noise arguments");
   FDK_REQUIRE(n >= 0, "negative count");
+  FDK_REQUIRE(i0 > 0 && mu > 0, "bad noise arguments (i0 %g, mu %g)", i0, mu);
   if (n == 0) return FDK_OK;
+  FDK_REQUIRE(dev_proj, "NULL projections");
   hipLaunchKernelGGL(poisson_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
                      dev_proj, (long long)n, i0, mu, seed);
   FDK_HIP(hipGetLastError());

@@ -1,16 +1,21 @@
 // Streaming reconstruction pipeline: host views in 32-view batches ->
-// host-to-device copy on a copy stream -> (cos weighting + ramp filter) ->
-// backprojection on a compute stream, with a ring of device batch slots so
-// that the copy of batch k+1 runs under the backprojection of batch k.
+// host-to-device copy on a copy stream -> (cos weighting + ramp filter) on a
+// filter stream -> backprojection on a compute stream, with a ring of device
+// batch slots so that the copy and the filtering of batch k+1 run under the
+// backprojection of batch k.
 //
 // The single-node counterpart of the reference's FilterThread / BpThread pair
 // joined by SafeList queues (cuFDK/mpiFDK.cpp:8-317, PushBpData 167-203): the
-// reference moves batches between host threads and makes each BP call
-// synchronous (cudaBP/backprojectionFDK.cu:229-276); here one host thread
-// enqueues, and HIP events order the copy, filter and backprojection of each
-// slot.  A push returns once its batch has been copied, so the caller may
-// refill its host buffer at once (page-locked buffers keep the copy
-// asynchronous to the other stream's kernels).
+// reference filters on the host in one thread while another makes synchronous
+// BP calls (cudaBP/backprojectionFDK.cu:229-276); here one host thread
+// enqueues, three streams carry the copy, the filter and the backprojection,
+// and HIP events order them per slot.  16-bit storage keeps each batch's
+// per-view min/max on the device, where the backprojector reads its scale and
+// offset, so nothing waits on the host between filter and backprojection.  A
+// push returns once its batch has been copied, so the caller may refill its
+// host buffer at once (page-locked buffers keep the copy asynchronous to the
+// other streams' kernels).  Device buffers are allocated at creation, the
+// raw-view buffers at the first raw push; later pushes allocate nothing.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -25,15 +30,18 @@ struct fdk_pipe {
   int store_dtype = FDK_F32;
   int pad_mode = FDK_PAD_MEAN4;
   int N = 0;
-  hipStream_t copy = nullptr, compute = nullptr;
+  hipStream_t copy = nullptr, filter = nullptr, compute = nullptr;
   float* d_cos = nullptr;
   float* d_filter = nullptr;
+  float* d_stage = nullptr;  // FP32 filtered batch before 16-bit quantisation (filter stream only)
   struct Slot {
-    void* raw = nullptr;    // [32][nv][nu] FP32, raw views (allocated on first raw push)
-    void* store = nullptr;  // [32][nu][nv] store dtype, filtered and transposed
-    hipEvent_t copied = nullptr, bp_start = nullptr, bp_end = nullptr, flt_start = nullptr;
+    void* raw = nullptr;       // [32][nv][nu] FP32, raw views (allocated on first raw push)
+    void* store = nullptr;     // [32][nu][nv] store dtype, filtered and transposed
+    unsigned* mm = nullptr;    // [32][2] per-view min/max words of a 16-bit filtered batch
+    hipEvent_t copied = nullptr, flt_start = nullptr, flt_end = nullptr, bp_start = nullptr,
+               bp_end = nullptr;
     bool pending = false;   // timing events recorded and not yet read
-    bool filtered = false;  // this use ran the filter (flt_start .. bp_start)
+    bool filtered = false;  // this use ran the filter (flt_start .. flt_end)
   };
   std::vector<Slot> slots;
   int next = 0;
@@ -53,7 +61,7 @@ int collect(fdk_pipe* p, fdk_pipe::Slot& s) {
   FDK_HIP(hipEventElapsedTime(&ms, s.bp_start, s.bp_end));
   p->bp_ms += ms;
   if (s.filtered) {
-    FDK_HIP(hipEventElapsedTime(&ms, s.flt_start, s.bp_start));
+    FDK_HIP(hipEventElapsedTime(&ms, s.flt_start, s.flt_end));
     p->filter_ms += ms;
   }
   s.pending = false;
@@ -64,17 +72,21 @@ int release(fdk_pipe* p) {
   if (!p) return 0;
   (void)hipSetDevice(p->dev);
   if (p->compute) (void)hipStreamSynchronize(p->compute);
+  if (p->filter) (void)hipStreamSynchronize(p->filter);
   if (p->copy) (void)hipStreamSynchronize(p->copy);
   for (auto& s : p->slots) {
     if (s.raw) (void)hipFree(s.raw);
     if (s.store) (void)hipFree(s.store);
-    for (hipEvent_t e : {s.copied, s.bp_start, s.bp_end, s.flt_start})
+    if (s.mm) (void)hipFree(s.mm);
+    for (hipEvent_t e : {s.copied, s.flt_start, s.flt_end, s.bp_start, s.bp_end})
       if (e) (void)hipEventDestroy(e);
   }
   if (p->bp) fdk_bp_destroy(p->bp);
   if (p->d_cos) (void)hipFree(p->d_cos);
   if (p->d_filter) (void)hipFree(p->d_filter);
+  if (p->d_stage) (void)hipFree(p->d_stage);
   if (p->copy) (void)hipStreamDestroy(p->copy);
+  if (p->filter) (void)hipStreamDestroy(p->filter);
   if (p->compute) (void)hipStreamDestroy(p->compute);
   delete p;
   return 0;
@@ -97,6 +109,7 @@ int setup(fdk_pipe* p, const fdk_geometry* g, const int64_t top[6], const int64_
   rc = fdk_bp_create(g->nv, g->nu, FDK_BP_MAX_PROJ, g->nx, g->ny, g->nz, top, bottom, &p->bp);
   if (rc < 0) return rc;
   FDK_HIP(hipStreamCreateWithFlags(&p->copy, hipStreamNonBlocking));
+  FDK_HIP(hipStreamCreateWithFlags(&p->filter, hipStreamNonBlocking));
   FDK_HIP(hipStreamCreateWithFlags(&p->compute, hipStreamNonBlocking));
   rc = fdk_bp_set_stream(p->bp, p->compute);
   if (rc < 0) return rc;
@@ -108,10 +121,10 @@ int setup(fdk_pipe* p, const fdk_geometry* g, const int64_t top[6], const int64_
   const size_t store_bytes = FDK_BP_MAX_PROJ * p->view_elems * esize(p->store_dtype);
   for (auto& s : p->slots) {
     FDK_HIP(hipMalloc(&s.store, store_bytes));
-    FDK_HIP(hipEventCreate(&s.copied));
-    FDK_HIP(hipEventCreate(&s.bp_start));
-    FDK_HIP(hipEventCreate(&s.bp_end));
-    FDK_HIP(hipEventCreate(&s.flt_start));
+    if (p->store_dtype != FDK_F32)
+      FDK_HIP(hipMalloc((void**)&s.mm, 2 * FDK_BP_MAX_PROJ * sizeof(unsigned)));
+    for (hipEvent_t* e : {&s.copied, &s.flt_start, &s.flt_end, &s.bp_start, &s.bp_end})
+      FDK_HIP(hipEventCreate(e));
   }
   return 0;
 }
@@ -146,31 +159,48 @@ int push(fdk_pipe* p, const void* host, int count, int first, int step, bool raw
 
   fdk_pipe::Slot& s = p->slots[p->next];
   p->next = (p->next + 1) % (int)p->slots.size();
-  rc = collect(p, s);  // waits until this slot's previous backprojection has finished
+  // Waits until this slot's previous backprojection (and so its filtering)
+  // has finished: the slot's buffers are then free to overwrite.
+  rc = collect(p, s);
   if (rc < 0) return rc;
   if (raw && !s.raw)
     FDK_HIP(hipMalloc(&s.raw, FDK_BP_MAX_PROJ * p->view_elems * sizeof(float)));
+  if (raw && p->store_dtype != FDK_F32 && !p->d_stage)
+    FDK_HIP(hipMalloc((void**)&p->d_stage, FDK_BP_MAX_PROJ * p->view_elems * sizeof(float)));
 
   const size_t bytes = count * p->view_elems * (raw ? sizeof(float) : esize(p->store_dtype));
   FDK_HIP(hipMemcpyAsync(raw ? s.raw : s.store, host, bytes, hipMemcpyHostToDevice, p->copy));
   FDK_HIP(hipEventRecord(s.copied, p->copy));
-  FDK_HIP(hipStreamWaitEvent(p->compute, s.copied, 0));
 
-  float so[2 * FDK_BP_MAX_PROJ];
-  const float* so_use = scale_offset;
   s.filtered = raw;
+  const bool dev_scale = raw && p->store_dtype != FDK_F32;
   if (raw) {
-    FDK_HIP(hipEventRecord(s.flt_start, p->compute));
-    rc = fdk_filter_views(static_cast<const float*>(s.raw), (int64_t)p->view_elems, s.store,
-                          (int64_t)p->view_elems, p->store_dtype, p->g.nu, p->g.nv, count,
-                          p->d_cos, p->d_filter, p->N, p->pad_mode, 1,
-                          p->store_dtype == FDK_F32 ? nullptr : so, p->compute);
+    // Filter stream: cos weighting + ramp filter + transposition (+ per-view
+    // min/max and quantisation for 16-bit storage) once the copy has landed.
+    FDK_HIP(hipStreamWaitEvent(p->filter, s.copied, 0));
+    FDK_HIP(hipEventRecord(s.flt_start, p->filter));
+    if (p->store_dtype == FDK_F32)
+      rc = fdk_filter_views(static_cast<const float*>(s.raw), (int64_t)p->view_elems, s.store,
+                            (int64_t)p->view_elems, FDK_F32, p->g.nu, p->g.nv, count, p->d_cos,
+                            p->d_filter, p->N, p->pad_mode, 1, nullptr, p->filter);
+    else
+      rc = fdk_filter_views_async(static_cast<const float*>(s.raw), (int64_t)p->view_elems,
+                                  s.store, (int64_t)p->view_elems, p->store_dtype, p->g.nu,
+                                  p->g.nv, count, p->d_cos, p->d_filter, p->N, p->pad_mode, 1,
+                                  p->d_stage, s.mm, p->filter);
     if (rc < 0) return rc;
-    so_use = p->store_dtype == FDK_F32 ? nullptr : so;
+    FDK_HIP(hipEventRecord(s.flt_end, p->filter));
+    FDK_HIP(hipStreamWaitEvent(p->compute, s.flt_end, 0));
+  } else {
+    FDK_HIP(hipStreamWaitEvent(p->compute, s.copied, 0));
   }
   FDK_HIP(hipEventRecord(s.bp_start, p->compute));
-  rc = fdk_bp_batch_device(p->bp, 1, s.store, p->store_dtype, p->g.nv, p->g.nu,
-                           (int64_t)p->view_elems, mats, so_use, count);
+  if (dev_scale)
+    rc = fdk_bp_batch_device_minmax(p->bp, s.store, p->store_dtype, p->g.nv, p->g.nu,
+                                    (int64_t)p->view_elems, mats, s.mm, count);
+  else
+    rc = fdk_bp_batch_device(p->bp, 1, s.store, p->store_dtype, p->g.nv, p->g.nu,
+                             (int64_t)p->view_elems, mats, raw ? nullptr : scale_offset, count);
   if (rc < 0) return rc;
   FDK_HIP(hipEventRecord(s.bp_end, p->compute));
   s.pending = true;
@@ -222,6 +252,7 @@ int fdk_pipe_finish(fdk_pipe_t p) {
   FDK_REQUIRE(p, "null pipeline");
   FDK_HIP(hipSetDevice(p->dev));
   FDK_HIP(hipStreamSynchronize(p->copy));
+  FDK_HIP(hipStreamSynchronize(p->filter));
   FDK_HIP(hipStreamSynchronize(p->compute));
   for (auto& s : p->slots) {
     const int rc = collect(p, s);

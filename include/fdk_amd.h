@@ -144,6 +144,15 @@ int fdk_bp_batch(fdk_bp_t h, int transpose, const float* host_proj, int width, i
 int fdk_bp_batch_device(fdk_bp_t h, int transpose, const void* dev_proj, int dtype, int width,
                         int height, int64_t view_stride, const float* mats,
                         const float* scale_offset, int nproj);
+/* As fdk_bp_batch_device for FDK_F16 / FDK_BF16 transposed views whose per-view
+ * {min, max} words (2*nproj, order-preserving uints as fdk_filter_views_async
+ * leaves them) stay on the device: scale = max - min and offset = min are
+ * formed in the kernel, so no host round trip sits between the filter and the
+ * backprojection (the reference's BpThread takes host batches,
+ * cuFDK/mpiFDK.cpp:245; this is the device form of that hand-over). */
+int fdk_bp_batch_device_minmax(fdk_bp_t h, const void* dev_proj, int dtype, int width, int height,
+                               int64_t view_stride, const float* mats,
+                               const unsigned* dev_minmax, int nproj);
 int fdk_bp_sync(fdk_bp_t h);
 /* Views backprojected so far (nBpCount) and summed kernel time (fKernelTime, ms;
  * only synchronous calls record it). */
@@ -202,6 +211,17 @@ int fdk_filter_views(const float* dev_src, int64_t view_stride_in, void* dev_dst
                      int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
                      const float* dev_cos, const float* dev_filter, int N, int pad_mode,
                      int transpose, float* host_scale_offset, void* stream);
+/* Asynchronous form of fdk_filter_views for FDK_F16 / FDK_BF16 output (the
+ * filter stage of the streaming pipeline, FilterThread in cuFDK/mpiFDK.cpp:58-154):
+ * the FP32 filtered batch goes through dev_workspace (nviews * nu * nv floats)
+ * and the per-view {min, max} stay in dev_minmax (2*nviews uints, for
+ * fdk_bp_batch_device_minmax); nothing synchronizes, and nothing is allocated
+ * except the FFT twiddle table of length N, built once per device and kept. */
+int fdk_filter_views_async(const float* dev_src, int64_t view_stride_in, void* dev_dst,
+                           int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
+                           const float* dev_cos, const float* dev_filter, int N, int pad_mode,
+                           int transpose, float* dev_workspace, unsigned* dev_minmax,
+                           void* stream);
 /* FilterProjImage: host views (nviews x nv x nu) filtered in place, in batches of
  * depth views (32 by default when depth <= 0), cos weighting by host_cos.
  * *elapsed_ms (when not NULL) receives the summed time of the batches' upload,

@@ -351,8 +351,10 @@ def _noise_unif(seed: int, idx: np.ndarray, k) -> np.ndarray:
     hi = ((i64 * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(32)).astype(np.uint32)
     lo = (i64 & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     kk = np.asarray(k, dtype=np.uint64) * np.uint64(0x632BE5AB)
-    h = _hash32(np.uint32(seed) ^ _hash32(hi)) ^ _hash32(
-        (lo.astype(np.uint64) + kk).astype(np.uint64).astype(np.uint32))
+    # hash32(hash32(seed ^ hi) ^ hash32(lo + 0x632BE5AB k)), all in 32-bit arithmetic
+    a = _hash32(np.uint32(seed) ^ hi)
+    c = _hash32((lo.astype(np.uint64) + kk).astype(np.uint32))
+    h = _hash32(a ^ c)
     return (h.astype(np.float64) + 0.5) * (1.0 / 4294967296.0)
 
 

@@ -1,0 +1,167 @@
+"""One GPU's share of a 4096^3 reconstruction: a 4096 x 4096 x 512 mirror slab
+pair (0.0667 mm voxels) from 4096^2-pixel views (0.1 mm) of an 8192-view scan,
+views streamed from page-locked host memory in 32-view batches through the
+pipeline (host-to-device copy, filter and backprojection on three streams).
+
+Sixty-four views of the scan (every 128th, pushed as two 32-view batches with
+view_step 256) of a seeded 80-ellipsoid phantom, into the central slab pair
+(z in [1792, 2048) and its mirror [2048, 2304), where the phantom is):
+
+* pushed raw with FP32 storage: about 3,000 sampled voxels agree with the
+  oracle FDK of the same views (mean-4 pad, FP32 filter, strict bilinear
+  backprojection with the same FP32 matrices, FP64 accumulation);
+* pushed raw with FP16 storage (filter, per-view min/max and quantisation on the
+  filter stream, the backprojector reading scale and offset from the device):
+  the same voxels agree with the oracle FDK of the oracle's filtered views
+  quantized the same way (per-view (p - min) / (max - min) to FP16), and the
+  error against the unquantized oracle is reported;
+* the same views filtered on the device, copied back to the host and pushed
+  pre-filtered with their host scale/offset give exactly the volume of the raw
+  FP16 push.
+"""
+import numpy as np
+import pytest
+
+import fdk_amd as F
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+NZS = 512
+VIEWS = 8192
+STEP = 256  # within a batch; the second batch starts 128 views later
+Z0 = N // 2 - NZS // 2  # the central slab pair: [Z0, N/2) and its mirror [N/2, N - Z0)
+
+
+def c5_geometry():
+    det = 0.1
+    vox = det * 1000.0 / 1500.0
+    return F.Parameter(nx=N, ny=N, nz=N, dx=vox, dy=vox, dz=vox, nu=N, nv=N, du=det, dv=det,
+                       sod=1000.0, sid=1500.0, nproj=VIEWS, direction=-1, scan_angle_deg=360.0,
+                       volume_position_angle_rad=0.0, filter_type=5)
+
+
+def rel_rmse(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def scan(torch_cuda):
+    torch = torch_cuda
+    p = c5_geometry()
+    ell = F.random_ellipsoids(80, seed=3, radius_mm=0.45 * N * p.dx,
+                              half_height_mm=0.45 * N * p.dz,
+                              axes_mm=(5.0 * p.dx / 0.1333, 100.0 * p.dx / 0.1333))
+    batches = [(0, STEP), (STEP // 2, STEP)]  # (first view, step) of each 32-view batch
+    views = [f + k * s for f, s in batches for k in range(32)]
+    print(f"C5: {N}^2 views, views {views[:3]}..{views[-1]}", flush=True)
+    raw = torch.empty((len(views), N, N), device="cuda")
+    for i, v in enumerate(views):
+        F.phantom_project(ell, p, v, 1, raw[i])
+    host_raw = torch.empty((len(views), N, N), dtype=torch.float32, pin_memory=True)
+    host_raw.copy_(raw)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(5)
+    k = 3000
+    zs = rng.integers(Z0, N // 2, k)
+    zs = np.where(rng.random(k) < 0.5, zs, N - 1 - zs)  # top slab or its mirror
+    ijk = np.stack([rng.integers(0, N, k), rng.integers(0, N, k), zs], 1).astype(np.int32)
+    return p, views, batches, raw, host_raw, ijk
+
+
+def boxes():
+    half = NZS // 2
+    return (0, 0, Z0, N, N, half), (0, 0, N // 2, N, N, half)
+
+
+def sample(torch, pipe, ijk):
+    """Voxels (x, y, z) of the [x][y][zz] working volume (zz: top slab, then
+    the bottom slab)."""
+    vol = pipe.bp.volume_tensor()
+    half = NZS // 2
+    z = ijk[:, 2]
+    zz = np.where(z < N // 2, z - Z0, half + (z - N // 2))
+    idx = [torch.as_tensor(a.astype(np.int64), device="cuda") for a in (ijk[:, 0], ijk[:, 1], zz)]
+    return vol[idx[0], idx[1], idx[2]].double().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def oracle_voxels(scan):
+    p, views, _, _, host_raw, ijk = scan
+    g = O.Geom(**{k: getattr(p, k) for k in O.Geom.__dataclass_fields__})
+    H = p.filter().astype(np.float32).astype(np.float64)
+    cos = p.cos_table()
+    raw = host_raw.numpy()
+    flt = np.empty_like(raw)
+    for b in range(0, len(views), 8):
+        flt[b:b + 8] = O.filter_rows(O.cos_weight(raw[b:b + 8], cos), H, O.PAD_MEAN4)
+    mats = p.proj_matrices().astype(np.float32).astype(np.float64)[views]
+    ref = O.bp_voxels(g, flt, mats, ijk, O.BP_GPU_STRICT)
+    # the same views stored as FP16: q = (p - min) / (max - min) per view
+    for v in range(len(views)):
+        lo, hi = np.float32(flt[v].min()), np.float32(flt[v].max())
+        inv = np.float32(1.0) / (hi - lo)
+        q = ((flt[v] - lo) * inv).astype(np.float16).astype(np.float32)
+        flt[v] = q * (hi - lo) + lo
+    ref16 = O.bp_voxels(g, flt, mats, ijk, O.BP_GPU_STRICT)
+    return {"f32": ref, "f16": ref16}
+
+
+def run_raw(torch, scan, store_dtype):
+    p, views, batches, _, host_raw, ijk = scan
+    pipe = F.Pipeline(p, *boxes(), store_dtype=store_dtype, slots=2)
+    for b, (first, step) in enumerate(batches):
+        pipe.push_raw(host_raw[32 * b:32 * b + 32], 32, first, step)
+    pipe.finish()
+    st = pipe.stats()
+    assert st["views"] == 64 and st["filter_ms"] > 0 and st["bp_ms"] > 0
+    assert st["h2d_bytes"] == 64 * N * N * 4
+    print(f"  stats {st}", flush=True)
+    return pipe
+
+
+def test_c5_raw_f32_matches_oracle(torch_cuda, scan, oracle_voxels):
+    torch = torch_cuda
+    print("C5 raw FP32 push", flush=True)
+    pipe = run_raw(torch, scan, F.FDK_F32)
+    got = sample(torch, pipe, scan[5])
+    pipe.close()
+    err = rel_rmse(got, oracle_voxels["f32"])
+    print(f"  rel RMSE vs oracle on {len(got)} voxels: {err:.2e}", flush=True)
+    assert err < 2e-5
+
+
+def test_c5_fp16_raw_and_filtered_pushes(torch_cuda, scan, oracle_voxels):
+    torch = torch_cuda
+    p, views, batches, raw, _, ijk = scan
+    print("C5 raw FP16 push", flush=True)
+    pipe = run_raw(torch, scan, F.FDK_F16)
+    got = sample(torch, pipe, ijk)
+    vol_raw = pipe.bp.volume_tensor().clone()
+    pipe.close()
+    err = rel_rmse(got, oracle_voxels["f16"])
+    err32 = rel_rmse(got, oracle_voxels["f32"])
+    print(f"  rel RMSE vs the FP16-quantized oracle {err:.2e}; vs the FP32 oracle {err32:.2e}",
+          flush=True)
+    assert err < 1e-4
+    assert err32 < 5e-3
+
+    print("C5 filtered FP16 push", flush=True)
+    cos = torch.as_tensor(p.cos_table()).cuda()
+    H = torch.as_tensor(p.filter().astype(np.float32)).cuda()
+    flt = torch.empty((64, N, N), dtype=torch.int16, device="cuda")
+    so = F.filter_views(raw, flt, cos, H, F.PAD_MEAN4, True, dtype=F.FDK_F16)
+    host_flt = torch.empty((64, N, N), dtype=torch.int16, pin_memory=True)
+    host_flt.copy_(flt)
+    torch.cuda.synchronize()
+    del flt
+    pipe = F.Pipeline(p, *boxes(), store_dtype=F.FDK_F16, slots=3)
+    for b, (first, step) in enumerate(batches):
+        pipe.push_filtered(host_flt[32 * b:32 * b + 32], 32, first, step, so[32 * b:32 * b + 32])
+    pipe.finish()
+    st = pipe.stats()
+    assert st["views"] == 64 and st["filter_ms"] == 0 and st["h2d_bytes"] == 64 * N * N * 2
+    same = bool(torch.equal(pipe.bp.volume_tensor(), vol_raw))
+    pipe.close()
+    assert same, "filtered push differs from the raw push of the same views"

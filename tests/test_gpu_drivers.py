@@ -55,7 +55,9 @@ def test_cpp_forward_projection_driver_matches_oracle(tmp_path):
     vol = np.fromfile(prefix + ".vol.raw", np.float32).reshape(n, n, n)
     fp = np.fromfile(prefix + ".fp.raw", np.float32).reshape(views, n, n)
     g = geom(n, views)
-    ref = np.stack([O.fp(g, vol, s, 0.5 * g.dx, O.FP_GPU_CLAMP) for s in range(views)])
+    # the reference signature marches at min(dx, dy, dz) (genProj's projector step)
+    step = min(g.dx, g.dy, g.dz)
+    ref = np.stack([O.fp(g, vol, s, step, O.FP_GPU_CLAMP) for s in range(views)])
     close = np.abs(fp - ref) <= 1e-4 * np.abs(ref).max()
     assert close.mean() > 0.995
 

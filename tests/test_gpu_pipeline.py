@@ -107,7 +107,7 @@ def test_pipeline_rejects_bad_pushes(torch_cuda):
     with pytest.raises(F.FdkError, match="outside"):
         pipe.push_raw(x, 8, 36)
     with pytest.raises(F.FdkError, match="scale_offset"):
-        pipe.push_filtered(x, 4, 0)
+        pipe.push_filtered(np.zeros((4, p.nu, p.nv), np.int16), 4, 0)
     with pytest.raises(F.FdkError, match="slots"):
         F.Pipeline(p, slots=7)
     pipe.push_raw(x[:8], 8, 32)  # a good push still works after rejected ones
