@@ -215,6 +215,7 @@ def precision_gpu(p, ell, bp, cos, H, steps: int = 8):
     rawv = torch.empty((V, p.nv, p.nu), device="cuda")
     F.phantom_project(ell, ps, 0, V, rawv)
     mats_s = ps.proj_matrices().astype(np.float32)
+    cos = torch.as_tensor(ps.cos_table()).cuda()  # carries the 64-view angular factor
     vols = {}
     half = sub // 2
     for name, dt in (("f32", F.FDK_F32), ("f16", F.FDK_F16), ("bf16", F.FDK_BF16)):
@@ -736,12 +737,13 @@ def main():
     avg_ms = float(np.mean(kern_ms))
     s_bytes = 4 if dtype == F.FDK_F32 else 2
     b_upd = 4 * s_bytes + 8.0 / B          # 4 taps + volume read/write per 32-view pass
-    # the library's choice for this workload: FP32 views use the prefetching
-    # LDS kernel, 16-bit views the LDS kernel; FDK_BP_KERNEL=v2 / v3 force the
-    # direct-gather / non-prefetching kernels
+    # the library's choice for this workload (a mirror-symmetric slab pair):
+    # the wave-strip kernel; FDK_BP_KERNEL=v4 / v3 / v2 force the block-strip
+    # prefetching, block-strip and direct-gather kernels
     forced = os.environ.get("FDK_BP_KERNEL")
-    kernel = ("bp_column_kernel" if forced == "v2" else
-              "bp_lds_kernel" if forced == "v3" or dtype != F.FDK_F32 else "bp_lds_pf_kernel")
+    kernel = {"v2": "bp_column_kernel", "v3": "bp_lds_kernel",
+              "v4": "bp_lds_pf_kernel" if dtype == F.FDK_F32 else "bp_lds_kernel"}.get(
+                  forced, "bp_wave_kernel")
     roofline = roofline_of(kernel, args.dtype, n, avg_ms, float(n) ** 3 * B, b_upd)
 
     checks = {}

@@ -738,6 +738,227 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
   if (active) column_write<SYM, ZPT>(a, xs, ys, zbase, acc, accm);
 }
 
+// ------------------------------------------------------------ wave-strip kernel
+// bp_wave_kernel: the same blocks, columns and lanes, for a mirror slab pair
+// whose matrices are mirror-symmetric (v(z) + v(nz-1-z) = W-1 for every
+// column, checked on the host), with no block-level synchronisation:
+//  * each wave stages, per view, ITS OWN u-interpolated detector strips in LDS:
+//    r(c) = w0 P[iu][c] + w1 P[iu+1][c] (w0, w1 the u weights times 1/w^2) over
+//    the v columns of its top z-run, and the same for the mirror run, which by
+//    symmetry reads columns W-1-c, stored reversed; top and mirror samples are
+//    interleaved, so one 16-byte LDS read brings r_top(iv), r_mir(iv),
+//    r_top(iv+1), r_mir(iv+1);
+//  * the mirror voxel has v' = W-1-v, so iv' = W-2-iv and t' = 1-t: its
+//    sample is r_mir(iv) + t (r_mir(iv+1) - r_mir(iv)) with the top voxel's iv
+//    and t, and the same validity (0 <= iv <= W-2);
+//  * the per-(column, view) values (u weights, v = Vb + z dV, strip bounds and
+//    flags) are computed once, lane s for view s, and broadcast with readlane;
+//  * the next view's strip loads are issued into registers before the current
+//    view is computed from LDS.
+// Per voxel pair: v, floor, fraction, 1-t, the LDS address (and the validity
+// test where the run crosses the detector edge) and four FMAs.
+constexpr int kWsQuads = 3;  // strip quads per lane and run (the host checks CT against it)
+
+// Per-view values of one column, kept in lane s for view s.
+struct WaveView {
+  float Vb, dV, w0, w1, Z2;
+  int iu;
+  int c0;     // first strip column (multiple of 4)
+  int quads;  // strip quads of 4 columns
+  int flags;  // 1: some voxel of the runs inside the view, 2: all inside, 4: the strip fits
+};
+
+template <int DT>
+__device__ __forceinline__ float4 ws_combine(typename Quad<DT>::raw A, typename Quad<DT>::raw B,
+                                             float w0, float w1, float zof) {
+  if constexpr (DT == FDK_F32) {
+    return make_float4(fmaf(w1, B.x, w0 * A.x), fmaf(w1, B.y, w0 * A.y), fmaf(w1, B.z, w0 * A.z),
+                       fmaf(w1, B.w, w0 * A.w));
+  } else {
+    // w0, w1 already carry the view's scale; zof = (w0 + w1) * offset
+    const float4 qa = Quad<DT>::cvt(A, 1.0f, 0.0f), qb = Quad<DT>::cvt(B, 1.0f, 0.0f);
+    return make_float4(fmaf(w1, qb.x, fmaf(w0, qa.x, zof)), fmaf(w1, qb.y, fmaf(w0, qa.y, zof)),
+                       fmaf(w1, qb.z, fmaf(w0, qa.z, zof)), fmaf(w1, qb.w, fmaf(w0, qa.w, zof)));
+  }
+}
+
+template <int DT>
+struct WsPre {
+  typename Quad<DT>::raw ta[kWsQuads], tb[kWsQuads], ma[kWsQuads], mb[kWsQuads];
+};
+
+// Strip loads of one view: quad q of the top run reads columns c0+4q..c0+4q+3
+// of rows iu and iu+1; of the mirror run, columns W-4-c0-4q..W-1-c0-4q.
+// Columns outside the row read other samples of the view or (negative or past
+// the view) zero through the buffer range check; only columns of valid taps
+// are ever used.
+template <int DT>
+__device__ __forceinline__ void ws_issue(WsPre<DT>& pre, __amdgpu_buffer_rsrc_t rv, int W, int iu,
+                                         int c0, int quads, int lane) {
+  const int es = DT == FDK_F32 ? 4 : 2;
+  const int rowa = iu * W * es, rowb = rowa + W * es;
+#pragma unroll
+  for (int k = 0; k < kWsQuads; ++k) {
+    const int q = lane + kWave * k;
+    if (q < quads) {
+      const int ct = (c0 + 4 * q) * es, cm = (W - 4 - c0 - 4 * q) * es;
+      pre.ta[k] = Quad<DT>::load(rv, rowa + ct);
+      pre.tb[k] = Quad<DT>::load(rv, rowb + ct);
+      pre.ma[k] = Quad<DT>::load(rv, rowa + cm);
+      pre.mb[k] = Quad<DT>::load(rv, rowb + cm);
+    }
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ void ws_commit(const WsPre<DT>& pre, float2* strip, float w0, float w1,
+                                          float zof, int quads, int lane) {
+#pragma unroll
+  for (int k = 0; k < kWsQuads; ++k) {
+    const int q = lane + kWave * k;
+    if (q < quads) {
+      const float4 t = ws_combine<DT>(pre.ta[k], pre.tb[k], w0, w1, zof);
+      const float4 m = ws_combine<DT>(pre.ma[k], pre.mb[k], w0, w1, zof);
+      float4* d = reinterpret_cast<float4*>(strip + 4 * q);
+      d[0] = make_float4(t.x, m.w, t.y, m.z);  // mirror quad reversed
+      d[1] = make_float4(t.z, m.y, t.w, m.x);
+    }
+  }
+}
+
+template <bool CHECK, int ZPT>
+__device__ __forceinline__ void ws_compute(float (&acc)[ZPT], float (&accm)[ZPT],
+                                           const float (&zt)[ZPT], float Vb, float dV,
+                                           const float2* base, const float2* zslot, int W) {
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    const float v = fmaf(zt[j], dV, Vb);
+    const int iv = floor_int(v);
+    const float t = __builtin_amdgcn_fractf(v);
+    const float2* p = base + iv;
+    if (CHECK) p = (unsigned)iv <= (unsigned)(W - 2) ? p : zslot;
+    const float2 a = p[0], b = p[1];
+    const float omt = 1.0f - t;
+    acc[j] = fmaf(t, b.x, fmaf(omt, a.x, acc[j]));
+    accm[j] = fmaf(t, b.y, fmaf(omt, a.y, accm[j]));
+  }
+}
+
+__device__ __forceinline__ float lane_f(float v, int s) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), s));
+}
+__device__ __forceinline__ int lane_i(int v, int s) { return __builtin_amdgcn_readlane(v, s); }
+
+template <int DT, int ZPT>
+__global__ __launch_bounds__(256) void bp_wave_kernel(BpArgs a, BpMats M, BpScale S, int CT) {
+  extern __shared__ float2 wstrip[];  // per wave: CT float2, the last two the zero slot
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int lane = threadIdx.x % kWave;
+  const int tile = lb / (kTile * kTile);
+  const int within = lb % (kTile * kTile);
+  const int px = (tile % a.tiles_x) * kTile + within % kTile;
+  const int py = (tile / a.tiles_x) * kTile + within / kTile;
+  const int xs = 2 * px + (wave & 1);
+  const int ys = 2 * py + (wave >> 1);
+  if (xs >= a.nxs || ys >= a.nys) return;  // no block barrier: the wave just leaves
+  float2* strip = wstrip + wave * CT;
+  const float2* zslot = strip + CT - 2;
+  if (lane < 2) strip[CT - 2 + lane] = make_float2(0.f, 0.f);
+  const int zb0 = blockIdx.y * (kWave * ZPT);
+  const int zbase = zb0 + lane;
+  const float x = (float)(a.x0 + xs);
+  const float y = (float)(a.y0 + ys);
+  float acc[ZPT], accm[ZPT], zt[ZPT];
+#pragma unroll
+  for (int j = 0; j < ZPT; ++j) {
+    acc[j] = 0.f;
+    accm[j] = 0.f;
+    zt[j] = (float)(a.z0 + zbase + j * kWave);
+  }
+
+  // lane s: the values of view s for this column
+  WaveView wv{};
+  if (lane < a.nproj) {
+    const float* m = M.m[lane];
+    const float wd = fmaf(m[8], x, fmaf(m[9], y, m[11]));
+    const float Z = 1.0f / wd;
+    const float U = fmaf(m[0], x, fmaf(m[1], y, m[3])) * Z;
+    wv.Vb = fmaf(m[4], x, fmaf(m[5], y, m[7])) * Z;
+    wv.dV = m[6] * Z;
+    wv.Z2 = Z * Z;
+    const float fu0 = floorf(U);
+    wv.iu = (int)fu0;
+    const bool uok = (unsigned)wv.iu <= (unsigned)(a.H - 2);
+    const float fu = U - fu0;
+    wv.w0 = uok ? (1.0f - fu) * wv.Z2 : 0.0f;
+    wv.w1 = uok ? fu * wv.Z2 : 0.0f;
+    // v over the top run, as the compute evaluates it (monotone in z)
+    const float va = fmaf((float)(a.z0 + zb0), wv.dV, wv.Vb);
+    const float vb = fmaf((float)(a.z0 + zb0 + kWave * ZPT - 1), wv.dV, wv.Vb);
+    const int ilo = floor_int(fminf(va, vb)), ihi = floor_int(fmaxf(va, vb));
+    const bool any = uok && ihi >= 0 && ilo <= a.W - 2;
+    const bool all = uok && ilo >= 0 && ihi <= a.W - 2;
+    wv.c0 = max(ilo, 0) & ~3;
+    const int c1 = min(ihi + 1, a.W - 1);
+    wv.quads = any ? (c1 - wv.c0 + 4) >> 2 : 0;
+    const bool fits = 4 * wv.quads <= CT - 2 && wv.quads <= kWave * kWsQuads;
+    wv.flags = (any ? 1 : 0) | (all ? 2 : 0) | (fits ? 4 : 0);
+  }
+
+  WsPre<DT> pre;
+  int f_n = lane_i(wv.flags, 0);
+  if ((f_n & 5) == 5)
+    ws_issue<DT>(pre, view_rsrc(a, 0), a.W, lane_i(wv.iu, 0), lane_i(wv.c0, 0),
+                 lane_i(wv.quads, 0), lane);
+  for (int s = 0; s < a.nproj; ++s) {
+    const int f = f_n;
+    const float Vb = lane_f(wv.Vb, s), dV = lane_f(wv.dV, s);
+    float w0 = lane_f(wv.w0, s), w1 = lane_f(wv.w1, s);
+    const int c0 = lane_i(wv.c0, s), quads = lane_i(wv.quads, s);
+    float zof = 0.f;
+    float2 so = make_float2(1.f, 0.f);
+    if (DT != FDK_F32) {
+      so = view_so(a, S, s);
+      zof = (w0 + w1) * so.y;
+    }
+    if ((f & 5) == 5)
+      ws_commit<DT>(pre, strip, DT == FDK_F32 ? w0 : w0 * so.x, DT == FDK_F32 ? w1 : w1 * so.x, zof,
+                    quads, lane);
+    if (s + 1 < a.nproj) {
+      f_n = lane_i(wv.flags, s + 1);
+      if ((f_n & 5) == 5)
+        ws_issue<DT>(pre, view_rsrc(a, s + 1), a.W, lane_i(wv.iu, s + 1), lane_i(wv.c0, s + 1),
+                     lane_i(wv.quads, s + 1), lane);
+    }
+    if (!(f & 1)) continue;
+    if (f & 4) {
+      const float2* base = strip - c0;
+      if (f & 2)
+        ws_compute<false, ZPT>(acc, accm, zt, Vb, dV, base, zslot, a.W);
+      else
+        ws_compute<true, ZPT>(acc, accm, zt, Vb, dV, base, zslot, a.W);
+    } else {
+      // strip too wide for the LDS slot: the taps straight from memory
+      ColView c;
+      c.Vb = Vb;
+      c.dV = dV;
+      c.w0 = w0;
+      c.w1 = w1;
+      c.Z2 = lane_f(wv.Z2, s);
+      c.iu = lane_i(wv.iu, s);
+      c.uok = true;
+      float zm[ZPT];
+#pragma unroll
+      for (int j = 0; j < ZPT; ++j) zm[j] = (float)(a.nz - 1) - zt[j];
+      compute_direct<DT, true, ZPT>(acc, accm, zt, zm, c, a,
+                                    static_cast<const char*>(a.proj) + (size_t)s * a.view_bytes,
+                                    so.x, so.y);
+    }
+  }
+  column_write<true, ZPT>(a, xs, ys, zbase, acc, accm);
+}
+
 // General matrices (a z term in rows 0 or 2): every voxel does the full
 // projective map.  Same layout and sampling as bp_column_kernel, ZPT = 1.
 template <int DT, bool SYM>
@@ -933,16 +1154,60 @@ bool stage_dims(const BpArgs& a, const BpMats& M, int zpt, int* R, int* CT) {
 }
 
 // Kernel choice: FDK_BP_KERNEL=v2 selects the direct-gather bp_column_kernel,
-// v3 the LDS-staged bp_lds_kernel; by default bp_lds_pf_kernel runs when its
-// register prefetch covers the strip, else bp_lds_kernel, else the direct one.
+// v3 the LDS-staged bp_lds_kernel, v4 bp_lds_pf_kernel where its register
+// prefetch covers the strip; by default bp_wave_kernel runs for mirror-symmetric
+// slab pairs whose strips fit, then the v4 / v3 / v2 chain.
 int kernel_choice() {
   static const int k = [] {
     const char* e = std::getenv("FDK_BP_KERNEL");
     if (e && std::strcmp(e, "v2") == 0) return 2;
     if (e && std::strcmp(e, "v3") == 0) return 3;
-    return 4;
+    if (e && std::strcmp(e, "v4") == 0) return 4;
+    return 5;
   }();
   return k;
+}
+
+// v(z) + v(nz-1-z) = W-1 for every column of the sub-volume and every view, to
+// 1e-3 pixel (FP64 evaluation of the FP32 matrices at the corners; the error is
+// affine in x and y): the mirror voxel's sample position is then W-1-v.
+bool mirror_symmetric(const BpArgs& a, const BpMats& M) {
+  const double xs[2] = {(double)a.x0, (double)(a.x0 + a.nxs - 1)};
+  const double ys[2] = {(double)a.y0, (double)(a.y0 + a.nys - 1)};
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    for (double x : xs)
+      for (double y : ys) {
+        const double w = (double)m[8] * x + (double)m[9] * y + (double)m[11];
+        if (!(w > 0.0)) return false;
+        const double e = (2.0 * ((double)m[4] * x + (double)m[5] * y + (double)m[7]) +
+                          (double)m[6] * (a.nz - 1)) / w - (a.W - 1);
+        if (!(std::fabs(e) <= 1e-3)) return false;
+      }
+  }
+  return true;
+}
+
+// Strip slot CT (float2 per column, plus the two-entry zero slot) of
+// bp_wave_kernel: the v extent of one wave's z-run (64 * ZPT voxels) over the
+// views and the sub-volume's corners, with room for floor rounding and the
+// 4-column alignment of its start.  0 when a corner is behind the source.
+int wave_strip_ct(const BpArgs& a, const BpMats& M, int zpt) {
+  const double xs[2] = {(double)a.x0, (double)(a.x0 + a.nxs - 1)};
+  const double ys[2] = {(double)a.y0, (double)(a.y0 + a.nys - 1)};
+  double dvz = 0.0;
+  for (int s = 0; s < a.nproj; ++s) {
+    const float* m = M.m[s];
+    for (double x : xs)
+      for (double y : ys) {
+        const double w = (double)m[8] * x + (double)m[9] * y + (double)m[11];
+        if (!(w > 0.0)) return 0;
+        dvz = std::max(dvz, std::fabs((double)m[6] / w));
+      }
+  }
+  const double span = dvz * (kWave * zpt - 1);
+  const int quads = ((int)std::ceil(span) + 9) / 4 + 1;
+  return 4 * quads + 2;
 }
 
 template <int DT, int ZPT>
@@ -973,7 +1238,23 @@ bool launch_lds(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, u
 }
 
 template <int DT, int ZPT>
+bool launch_wave(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
+  if (kernel_choice() < 5 || !h->sym || a.W % 4 != 0) return false;
+  if (DT != FDK_F32 && ((reinterpret_cast<uintptr_t>(a.proj) % 8) != 0 || a.view_stride % 4 != 0))
+    return false;
+  if (!mirror_symmetric(a, M)) return false;
+  const int CT = wave_strip_ct(a, M, ZPT);
+  if (CT <= 0 || (CT - 2) / 4 > kWave * kWsQuads) return false;
+  const size_t lds = (size_t)4 * CT * sizeof(float2);
+  if (lds > 64 * 1024) return false;
+  dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
+  hipLaunchKernelGGL((bp_wave_kernel<DT, ZPT>), grid, dim3(256), lds, h->stream, a, M, S, CT);
+  return true;
+}
+
+template <int DT, int ZPT>
 void launch_column(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, unsigned nblk) {
+  if (launch_wave<DT, ZPT>(h, a, M, S, nblk)) return;
   if (launch_lds<DT, ZPT>(h, a, M, S, nblk)) return;
   dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));
   if (h->sym) hipLaunchKernelGGL((bp_column_kernel<DT, true, ZPT>), grid, dim3(256), 0, h->stream, a, M, S);
