@@ -100,6 +100,9 @@ _SIGS = {
     "fdk_filter_views": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64] + [C.c_int] * 4
                          + [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                             C.c_void_p]),
+    "fdk_filter_views_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+                                 + [C.c_int] * 4 + [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                                    C.c_int, C.c_void_p, C.c_void_p]),
     "fdk_filter_views_async": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
                                + [C.c_int] * 4 + [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -595,11 +598,14 @@ def FilterRow_Real32(src, dst, filt, pad_mode: int = PAD_MEAN3, stream=None):
 
 def filter_views(src, dst, cos_table, filt, pad_mode: int = PAD_MEAN4, transpose: bool = True,
                  dtype: int = FDK_F32, stream=None, nviews: int | None = None,
-                 src_view_offset: int = 0, dst_view_offset: int = 0, src_view_step: int = 1):
+                 src_view_offset: int = 0, dst_view_offset: int = 0, src_view_step: int = 1,
+                 no_alloc: bool = False):
     """Fused cos weight + row filter.  src (views, nv, nu) float32 device tensor;
     dst (views, nu, nv) when transposed else (views, nv, nu), of the stored dtype.
     ``src_view_step`` reads every k-th source view (a sparse-view subset) without a
-    copy.  Returns per-view (scale, offset) for FP16/BF16, else None."""
+    copy.  ``no_alloc`` runs fdk_filter_views_noalloc (no device allocation; at
+    most 1024 views and N <= 16384 per call).  Returns per-view (scale, offset)
+    for FP16/BF16, else None."""
     nv, nu = src.shape[-2], src.shape[-1]
     if nviews is None:
         nviews = (src.shape[0] - src_view_offset + src_view_step - 1) // src_view_step
@@ -619,7 +625,8 @@ def filter_views(src, dst, cos_table, filt, pad_mode: int = PAD_MEAN4, transpose
         _dev_buffer(cos_table, "filter_views cos table", view * 4)
     _dev_buffer(filt, "filter_views filter", 0)
     so = np.zeros((max(n, 1), 2), np.float32) if dtype != FDK_F32 else None
-    _check(lib().fdk_filter_views(sp, view * int(src_view_step), dp, view, int(dtype), nu, nv, int(n),
+    fn = lib().fdk_filter_views_noalloc if no_alloc else lib().fdk_filter_views
+    _check(fn(sp, view * int(src_view_step), dp, view, int(dtype), nu, nv, int(n),
                                   _dev_ptr(cos_table) if cos_table is not None else None,
                                   _dev_ptr(filt), int(filt.numel()), int(pad_mode),
                                   1 if transpose else 0,

@@ -902,3 +902,104 @@ extern "C" int fdk_filter_frame_host(float* host_src, float* host_dst, int nu, i
   if (h) (void)hipFree(h);
   return r;
 }
+
+// ------------------------------------------------------------ no-allocation filter
+// fdk_filter_views without any allocation: the 16-bit forms run the filter
+// twice (the first pass only forms each view's min/max, the second quantizes
+// as it stores) instead of staging the FP32 batch, and the min/max words and
+// the twiddle table live in `scratch`, a fixed device array of
+// kScratchBytes per device.
+namespace {
+constexpr int kNoAllocViews = 1024;  // views per call
+constexpr int kNoAllocN = 16384;     // FFT length
+constexpr size_t kScratchMinmax = 2 * kNoAllocViews * sizeof(unsigned);
+constexpr size_t kScratchBytes = kScratchMinmax + kNoAllocN * sizeof(float2);  // 136 KiB
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+// Per-view {min, max} words, then the twiddle table of the call's N.  Each
+// device holds its own copy for the life of the process, shared by every call
+// on that device.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// The device address of the current device's `scratch`, looked up on the
+// first call on that device (the array's name is not a device address in host
+// code).
+int scratch_ptr(unsigned char** out) {
+  int dev = 0;
+  FDK_HIP(hipGetDevice(&dev));
+  FDK_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fdk::fail(FDK_ERR_HIP, "address of the filter scratch: %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[dev];
+  return FDK_OK;
+}
+}  // namespace
+
+extern "C" int fdk_filter_views_noalloc(const float* dev_src, int64_t view_stride_in,
+                                        void* dev_dst, int64_t view_stride_out, int dtype, int nu,
+                                        int nv, int nviews, const float* dev_cos,
+                                        const float* dev_filter, int N, int pad_mode,
+                                        int transpose, float* host_scale_offset, void* stream) {
+  if (int r = check_views(dev_src, dev_dst, dev_filter, transpose, dtype, nu, nv, view_stride_in,
+                          view_stride_out))
+    return r;
+  FDK_REQUIRE(nviews >= 0 && nviews <= kNoAllocViews,
+              "%d views outside the no-allocation filter's 0..%d per call", nviews, kNoAllocViews);
+  FDK_REQUIRE(N >= 16 && N <= kNoAllocN, "filter length %d outside the no-allocation filter's "
+              "16..%d", N, kNoAllocN);
+  FDK_REQUIRE(dtype == FDK_F32 || host_scale_offset != nullptr,
+              "FP16/BF16 output needs host_scale_offset");
+  if (dtype != FDK_F32 && nviews > 0) {
+    // the second pass reads the source again: the 16-bit output must not overlap it
+    const char* s0 = reinterpret_cast<const char*>(dev_src);
+    const char* s1 = s0 + ((size_t)(nviews - 1) * view_stride_in + (size_t)nu * nv) * 4;
+    const char* d0 = static_cast<const char*>(dev_dst);
+    const char* d1 = d0 + ((size_t)(nviews - 1) * view_stride_out + (size_t)nu * nv) * 2;
+    FDK_REQUIRE(d1 <= s0 || d0 >= s1, "16-bit output overlaps the source views");
+  }
+  if (nviews == 0) return FDK_OK;
+  unsigned char* base = nullptr;
+  if (int r = scratch_ptr(&base)) return r;
+  unsigned* mm = reinterpret_cast<unsigned*>(base);
+  float2* tw = reinterpret_cast<float2*>(base + kScratchMinmax);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  launch_twiddles(tw, N, st);  // the last call's table may be of another N
+  FilterArgs a = view_args(dev_src, view_stride_in, nu, nv, dev_cos, dev_filter, N, pad_mode,
+                           transpose);
+  a.tw = tw;
+  a.dst = static_cast<float*>(dev_dst);
+  a.dst_view = view_stride_out;
+  if (dtype == FDK_F32) return launch_filter(a, nviews, st);
+  hipLaunchKernelGGL(minmax_init_kernel, dim3((nviews + 255) / 256), dim3(256), 0, st, mm, nviews);
+  a.qmode = kStoreNone;
+  a.minmax = mm;
+  if (int r = launch_filter(a, nviews, st)) return r;
+  a.qmode = dtype == FDK_F16 ? kStoreF16 : kStoreBF16;
+  a.qmm = mm;
+  a.minmax = nullptr;
+  if (int r = launch_filter(a, nviews, st)) return r;
+  // the words land in the caller's array and are turned into {scale, offset} there
+  hipError_t e = hipMemcpyAsync(host_scale_offset, mm, 2 * sizeof(unsigned) * nviews,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fdk::fail(FDK_ERR_HIP, "min/max readback: %s", hipGetErrorString(e));
+  for (int v = 0; v < nviews; ++v) {
+    unsigned w[2];
+    std::memcpy(w, host_scale_offset + 2 * v, sizeof(w));
+    const float lo = host_ord2f(w[0]), hi = host_ord2f(w[1]);
+    host_scale_offset[2 * v] = hi - lo;
+    host_scale_offset[2 * v + 1] = lo;
+  }
+  return FDK_OK;
+}

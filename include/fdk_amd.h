@@ -222,6 +222,20 @@ int fdk_filter_views_async(const float* dev_src, int64_t view_stride_in, void* d
                            const float* dev_cos, const float* dev_filter, int N, int pad_mode,
                            int transpose, float* dev_workspace, unsigned* dev_minmax,
                            void* stream);
+/* fdk_filter_views without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc, code under stream capture for the FP32 form).  Same
+ * arguments, results and errors as fdk_filter_views, with limits nviews <= 1024
+ * and N <= 16384 per call (beyond them: FDK_ERR_INVALID).  It uses a fixed
+ * 136 KiB device array held by each device the library runs on until the
+ * process ends: the 16-bit forms filter twice (min/max, then quantized
+ * stores) instead of staging the FP32 batch, and then synchronize the stream
+ * to return host_scale_offset; the FP32 form only enqueues kernels.  Every call
+ * on a device shares that array, so calls that overlap on one device race. */
+int fdk_filter_views_noalloc(const float* dev_src, int64_t view_stride_in, void* dev_dst,
+                             int64_t view_stride_out, int dtype, int nu, int nv, int nviews,
+                             const float* dev_cos, const float* dev_filter, int N, int pad_mode,
+                             int transpose, float* host_scale_offset, void* stream);
 /* FilterProjImage: host views (nviews x nv x nu) filtered in place, in batches of
  * depth views (32 by default when depth <= 0), cos weighting by host_cos.
  * *elapsed_ms (when not NULL) receives the summed time of the batches' upload,
