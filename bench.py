@@ -600,6 +600,9 @@ def roofline_of(kernel: str, dt: str, n: int, avg_ms: float, upd_per_launch: flo
     l2 = c.get("l2_bytes_per_launch")
     r.update({
         "bound": "valu",
+        "bound_note": ("the busiest resource in the counter summary (VALU issue "
+                       f"{c['valu_busy']:.0%} busy, LDS {c['lds_busy']:.0%}, HBM "
+                       f"{c['hbm_bytes_per_launch'] / (c['gpu_cycles'] / 2.37e9) / HBM_PEAK:.0%})"),
         "achieved": round(valu / 1e9, 1),
         "peak": round(VALU_PEAK / 1e9, 1),
         "unit": "G VALU wave-instructions/s",
