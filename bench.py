@@ -54,6 +54,8 @@ def parse():
                     help="distinct host views, cycled through the scan")
     ap.add_argument("--stream-source", choices=["filtered", "raw"], default="filtered",
                     help="raw: FP32 raw views are streamed and filtered on the GPU")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end FDK of all views (filter + every pass)")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the streamed-slab leg of the default line")
     ap.add_argument("--stream-leg-views", type=int, default=256,
@@ -235,6 +237,47 @@ def precision_gpu(p, ell, bp, cos, H, steps: int = 8):
     return timing, {"geometry": ps, "vols": vols, "raw": rawv.cpu().numpy()}
 
 
+def e2e_fdk(p, ell, store, bp, mats, cos, H):
+    """The whole config-2 FDK timed end to end on the GPU: cos weighting and
+    ramp filtering of all views (raw FP32 views resident in HBM) and every
+    32-view backprojection pass, on one stream between two events; GUPS by
+    the reference's formula nx*nx*nz*views / t (cuFDK/mpiFDK.h:211)."""
+    import numpy as np
+    import torch
+    import fdk_amd as F
+    B = F.MAX_PROJ
+    V = p.nproj
+    raw_all = torch.empty((V, p.nv, p.nu), device="cuda")
+    for b in range(0, V, B):
+        F.phantom_project(ell, p, b, B, raw_all[b:b + B])
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(st)
+    F.filter_views(raw_all, store, cos, H, F.PAD_MEAN4, True, nviews=V)
+    ev[1].record(st)
+    bp.zero()
+    for b in range(0, V, B):
+        bp.bp_device(store, p.nv, p.nu, mats[b:b + B], B, offset_views=b)
+    ev[2].record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    del raw_all
+    vol = bp.volume_tensor()
+    probe = vol[p.nx // 2, p.ny // 2, ::97].float().cpu().numpy()
+    if not np.all(np.isfinite(probe)):
+        raise RuntimeError("non-finite voxels in the end-to-end volume")
+    t_f, t_b = ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
+    upd = float(p.nx) * p.nx * p.nz * V
+    return {"workload": f"{p.nx}^3 FDK from {V} views of {p.nu}x{p.nv}: filter every view, then "
+                        f"{V // B} backprojection passes, raw views resident in HBM",
+            "wall_s": round(wall, 3), "filter_s": round(t_f, 3), "bp_s": round(t_b, 3),
+            "gups": round(upd / wall / 1e9, 1),
+            "reference_context": "the reference's end-to-end 2048^3 FDK from 4096 views: 87.1 s "
+                                 "on 4 V100 (404 GUPS), 20.2 s on 64 V100 (1740 GUPS)"}
+
+
 def cpu_leg(p, frames_fn, seconds: float, checks: dict):
     """The CPU baseline leg: the reference's CPU path timed on this host's
     cores, and every comparison of GPU results with the oracle.
@@ -352,6 +395,25 @@ def cpu_leg(p, frames_fn, seconds: float, checks: dict):
             "against": "oracle FP64 strict-bilinear backprojection of the same filtered views "
                        "with the same FP32 matrices"}
 
+    # ---- the streamed slab's batch against FP64
+    sp = checks.get("stream_parity")
+    if sp is not None:
+        ps = sp["geometry"]
+        gs = O.Geom(**{k: getattr(ps, k) for k in O.Geom.__dataclass_fields__})
+        raw = sp["raw"]
+        f64 = np.empty_like(raw)
+        for b in range(0, raw.shape[0], 8):
+            f64[b:b + 8] = O.filter_rows(O.cos_weight(raw[b:b + 8], ps.cos_table()), ps.filter(),
+                                         O.PAD_MEAN4)
+        ref = O.bp_voxels(gs, f64, ps.proj_matrices(0, raw.shape[0]), sp["ijk"], O.BP_GPU_STRICT)
+        checks["stream_parity_result"] = {
+            "rel_rmse": rel_rmse(sp["got"], ref),
+            "max_abs_over_max": float(np.abs(sp["got"] - ref).max() / np.abs(ref).max()),
+            "voxels": int(len(ref)), "views": sp["views"],
+            "against": "FP64 reconstruction of the same raw views (oracle FP64 filter with FP64 "
+                       "H, FP64 matrices, FP64 strict-bilinear accumulation); the difference is "
+                       "the FP16 storage"}
+
     # ---- storage precision against FP64
     pr = checks.get("precision")
     if pr is not None:
@@ -445,7 +507,8 @@ def learned_sparse_view():
     return r
 
 
-def stream_leg(views: int, pool: int, source: str, warmup: int, log_steps: bool = False):
+def stream_leg(views: int, pool: int, source: str, warmup: int, log_steps: bool = False,
+               checks: dict | None = None):
     """The streamed slab: one GPU's share of a z-slab split of a 4096^3 volume
     (the central mirror pair, 512 slices) from 4096^2-pixel views of an
     8192-view scan, every view copied from page-locked host memory in 32-view
@@ -523,6 +586,23 @@ def stream_leg(views: int, pool: int, source: str, warmup: int, log_steps: bool 
     h2d = st["h2d_bytes"] - warm["h2d_bytes"]
     vol = pipe.bp.volume_tensor()
     probe = vol[n // 2, n // 2, ::31].float().cpu().numpy()
+    if checks is not None and raw_mode:
+        # parity sample: the first 32 views of the scan (the pool's first batch,
+        # real views at their own angles) pushed alone into a cleared slab
+        pipe.bp.zero()
+        pipe.finish()
+        push(0)
+        pipe.finish()
+        rng = np.random.default_rng(13)
+        k = 3000
+        zs = rng.integers(z0, n // 2, k)
+        zs = np.where(rng.random(k) < 0.5, zs, n - 1 - zs)
+        ijk = np.stack([rng.integers(0, n, k), rng.integers(0, n, k), zs], 1).astype(np.int32)
+        zz = np.where(zs < n // 2, zs - z0, nzs // 2 + (zs - n // 2))
+        idx = [torch.as_tensor(c.astype(np.int64), device=dev) for c in (ijk[:, 0], ijk[:, 1], zz)]
+        got = vol[idx[0], idx[1], idx[2]].double().cpu().numpy()
+        checks["stream_parity"] = {"geometry": p, "ijk": ijk, "got": got,
+                                   "raw": host[:B].numpy().copy(), "views": [0, B]}
     pipe.close()
     if not np.all(np.isfinite(probe)):
         raise RuntimeError("non-finite voxels in the streamed slab")
@@ -752,6 +832,7 @@ def main():
     checks = {}
     extra_legs = not args.no_cpu_baseline and dtype == F.FDK_F32
     precision = None
+    e2e = None
     if extra_legs:
         # parity of the timed kernel: the first timed pass again, alone, into a
         # cleared volume; sampled voxels go to the CPU leg
@@ -770,6 +851,10 @@ def main():
                             "frames": np.ascontiguousarray(
                                 store[s0:s0 + cnt].transpose(1, 2).cpu().numpy()),
                             "mats": mats[v0:v0 + cnt]}
+        if not args.no_e2e and resident == V:
+            log(f"end-to-end FDK: filter + backprojection of all {V} views")
+            e2e = e2e_fdk(p, ell, store, bp, mats, cos, H)
+            log(f"end-to-end: {e2e}")
         log("precision leg (FP16 / BF16 storage)")
         ptiming, checks["precision"] = precision_gpu(p, ell, bp, cos, H)
         precision = {"timing": ptiming}
@@ -777,6 +862,17 @@ def main():
         log("config 1 on the GPU")
         c1_timing, c1_vol, c1_raw = c1_gpu()
         checks["c1"] = {"timing": c1_timing, "vol": c1_vol, "raw": c1_raw}
+
+    stream_r = None
+    if not args.no_stream:
+        log("streamed slab leg (4096^2 x 512 from 4096^2 views, raw FP32 host views)")
+        try:
+            stream_r = stream_leg(args.stream_leg_views, args.stream_pool, "raw", 1,
+                                  checks=checks if extra_legs else None)
+        except Exception as e:  # reported, never decides the backprojection result
+            log(f"stream leg failed: {e!r}")
+            stream_r = {"error": repr(e)}
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -796,19 +892,11 @@ def main():
         precision["fp64_reference"] = ("central 256^3 of the 2048^3 geometry from every 64th of "
                                        "the 4096 views: oracle FP64 filter (FP64 H), FP64 "
                                        "matrices, FP64 strict-bilinear accumulation")
+    if stream_r is not None and "stream_parity_result" in checks:
+        stream_r["parity"] = checks["stream_parity_result"]
     bp.close()
     del store, raw, checks
     torch.cuda.empty_cache()
-
-    stream_r = None
-    if not args.no_stream:
-        log("streamed slab leg (4096^2 x 512 from 4096^2 views, raw FP32 host views)")
-        try:
-            stream_r = stream_leg(args.stream_leg_views, args.stream_pool, "raw", 1)
-        except Exception as e:  # reported, never decides the backprojection result
-            log(f"stream leg failed: {e!r}")
-            stream_r = {"error": repr(e)}
-        torch.cuda.empty_cache()
 
     learned = None
     if not args.no_learned:
@@ -840,6 +928,7 @@ def main():
         "roofline": roofline,
         "parity": parity,
         "cpu_baseline": cpu,
+        "end_to_end": e2e,
         "precision": precision,
         "stream": stream_r,
         "filter_stage": filter_stage,
