@@ -1,6 +1,9 @@
 // Error state and device queries of the C ABI (include/fdk_amd.h).
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -19,6 +22,37 @@ int fail(int code, const char* fmt, ...) {
   va_end(ap);
   g_last_error = buf;
   return code;
+}
+
+int stage_alloc(void** p, size_t bytes, hipStream_t st) {
+  static std::mutex mu;
+  static std::map<int, hipMemPool_t> pools;
+  int dev = 0;
+  FDK_HIP(hipGetDevice(&dev));
+  hipMemPool_t pool = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = pools.find(dev);
+    if (it != pools.end()) {
+      pool = it->second;
+    } else {
+      hipMemPoolProps props{};
+      props.allocType = hipMemAllocationTypePinned;
+      props.handleTypes = hipMemHandleTypeNone;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = dev;
+      FDK_HIP(hipMemPoolCreate(&pool, &props));
+      uint64_t keep = UINT64_MAX;  // freed blocks stay mapped for the next call
+      hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) {
+        (void)hipMemPoolDestroy(pool);
+        return fail(FDK_ERR_HIP, "staging pool: %s", hipGetErrorString(e));
+      }
+      pools[dev] = pool;
+    }
+  }
+  FDK_HIP(hipMallocFromPoolAsync(p, bytes, pool, st));
+  return FDK_OK;
 }
 
 }  // namespace fdk

@@ -21,6 +21,12 @@ constexpr int kRawRsrcWord3 = 0x00020000;
 // returns code, so call sites can `return fail(...)`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
+// Stream-ordered allocation from the library's per-device staging pool
+// (release with hipFreeAsync on a stream).  Freed blocks stay mapped in the
+// pool until the process ends, so a batch-sized staging buffer is mapped once,
+// not on every call.
+int stage_alloc(void** p, size_t bytes, hipStream_t st);
+
 }  // namespace fdk
 
 #define FDK_HIP(call)                                                                  \

@@ -15,12 +15,17 @@
 // MI355X design: one workgroup filters two rows at once as one complex
 // transform (row a in the real part, row b in the imaginary part).  Because H
 // is real and even, H * DFT(a + i b) inverts to filt(a) + i filt(b) exactly, so
-// no real-to-complex unpacking is needed.  The transform is an in-place
-// radix-4 (radix-2^2) decimation-in-time FFT in LDS (N complex floats, 64 KiB at
-// N = 8192), entered in bit-reversed order; the filter multiply doubles as the
-// bit-reversal permutation for the inverse pass.  Loads apply the cos table,
-// stores crop, optionally transpose (v fastest, what the backprojector reads)
-// and convert to the stored dtype.  Twiddles are an FP64-built table.
+// no real-to-complex unpacking is needed.  For N = 2^9 .. 2^13 the transform is
+// a Stockham plan of radix-16 (and radix-2/4/8) passes with compile-time
+// strides, the inverse running the forward radices in reverse order: each
+// thread keeps the points it loaded for the first pass, multiplies by H in
+// registers between the last forward and first inverse pass, and stores the
+// points its last inverse pass produced, so only the inner passes go through
+// LDS (N complex floats, 64 KiB at N = 8192).  Other lengths run an in-place
+// radix-2^2 FFT entered in bit-reversed order, the filter multiply doubling as
+// the inverse's bit-reversal.  Loads apply the cos table, stores crop,
+// optionally transpose (v fastest, what the backprojector reads) and convert to
+// the stored dtype.  Twiddles are an FP64-built table.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -46,7 +51,7 @@ struct FilterArgs {
   int transpose;          // 1: dst[view][u][row] with row fastest, pitch = nrows
   const float* cos_table; // nrows x nu (pitch nu) or null
   const float* H;         // N
-  const float2* tw;       // N/2 twiddles e^{-2 pi i k / N}
+  const float2* tw;       // N twiddles e^{-2 pi i k / N}
   int nu, nrows, N, logN, pad;
   unsigned* minmax;       // per view {min, max} as ordered uints, or null
   // Output mode: kStoreF32 writes dst as floats; kStoreNone writes nothing
@@ -223,33 +228,42 @@ constexpr int sk_span(int d) {
   return PAD ? d + d / 16 : d;
 }
 
-// One Stockham pass over N points in buf with BPT butterflies per thread
-// (stride NS between a butterfly's outputs); H (optional) multiplies the values
-// as they are read.  N, NS and the radix are compile-time, so every LDS access
-// is one per-thread base plus an immediate offset.
-template <int R, int BPT, bool PAD, int N, int NS>
-__device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* __restrict__ tw,
-                                        const float* __restrict__ H) {
+// Stockham passes over N points in buf with BPT butterflies per thread: a pass
+// of radix R reads the R values of butterfly j at stride NB = N / R, twiddles
+// them by e^{-2 pi i (j mod NS) r / (NS R)} (NS = the product of the radices
+// before it), runs a radix-R DFT in registers and writes them at stride NS.
+// N, NS and the radix are compile-time, so every LDS access is one per-thread
+// base plus an immediate offset.  Butterfly q of a thread is j = tid + 256 q;
+// a pass with fewer than 256 butterflies leaves the upper threads idle.
+template <int N, int R, int BPT>
+__device__ __forceinline__ bool sk_active(int q) {
+  constexpr int NB = N / R;
+  return NB >= 256 * (q + 1) || (int)threadIdx.x + q * 256 < NB;
+}
+
+template <int R, int BPT, bool PAD, int N>
+__device__ __forceinline__ void sk_read(const float2* buf, float2 (&v)[BPT][R]) {
   constexpr int NB = N / R;
   static_assert(NB % 16 == 0, "a read stride keeps the pad term separable");
   constexpr int kReadSpan = sk_span<PAD>(NB);
-  float2 v[BPT][R];
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
-    const int j = threadIdx.x + q * 256;
-    if (NB >= 256 * (q + 1) || j < NB) {
-      const float2* src = buf + sk_slot<PAD>(j);
+    if (sk_active<N, R, BPT>(q)) {
+      const float2* src = buf + sk_slot<PAD>(threadIdx.x + q * 256);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        float2 x = src[r * kReadSpan];
-        if (H) {
-          const float h = H[j + r * NB];
-          x = make_float2(x.x * h, x.y * h);
-        }
-        v[q][r] = x;
-      }
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * kReadSpan];
+    }
+  }
+}
+
+template <int R, int BPT, int N, int NS>
+__device__ __forceinline__ void sk_dft(float2 (&v)[BPT][R], float sign, const float2* __restrict__ tw) {
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    if (sk_active<N, R, BPT>(q)) {
       if constexpr (NS > 1) {
         // twiddle r of this butterfly: tw[step r], e^{-2 pi i step r / N}
+        const int j = threadIdx.x + q * 256;
         const unsigned step8 = (unsigned)((j & (NS - 1)) * (N / (NS * R))) * 8u;
         const __amdgpu_buffer_rsrc_t twr =
             __builtin_amdgcn_make_buffer_rsrc((void*)tw, 0, N * 8, fdk::kRawRsrcWord3);
@@ -265,11 +279,14 @@ __device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* _
       dft<R>(v[q], sign);
     }
   }
-  __syncthreads();
+}
+
+template <int R, int BPT, bool PAD, int N, int NS>
+__device__ __forceinline__ void sk_write(float2* buf, const float2 (&v)[BPT][R]) {
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
-    const int j = threadIdx.x + q * 256;
-    if (NB >= 256 * (q + 1) || j < NB) {
+    if (sk_active<N, R, BPT>(q)) {
+      const int j = threadIdx.x + q * 256;
       const int k = j & (NS - 1);
       const int base = (j - k) * R + k;
       if constexpr (NS == 1) {
@@ -287,42 +304,110 @@ __device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* _
       }
     }
   }
+}
+
+// One pass LDS -> LDS.
+template <int R, int BPT, bool PAD, int N, int NS>
+__device__ __forceinline__ void sk_pass(float2* buf, float sign, const float2* __restrict__ tw) {
+  float2 v[BPT][R];
+  sk_read<R, BPT, PAD, N>(buf, v);
+  sk_dft<R, BPT, N, NS>(v, sign, tw);
+  __syncthreads();  // every thread has read its inputs
+  sk_write<R, BPT, PAD, N, NS>(buf, v);
   __syncthreads();
 }
 
-// Radix plans for N = 2^M, 9 <= M <= 13 (256 threads); the LDS is padded for
-// M <= 12 (N + N/16 slots; at M = 13 that would pass 64 KiB).
+template <int N, int R>
+constexpr int sk_bpt() {
+  return N / R >= 256 ? N / R / 256 : 1;
+}
+
+// The whole filter on a thread's N/256 points, x[i] = point tid + 256 i on the
+// way in and out: forward passes of radices R1, R2, R3 (, R4), the filter
+// multiply, and inverse passes in the reverse radix order.  With that order
+// the points a thread loads are its first pass's inputs, its last forward
+// pass's outputs are its first inverse pass's inputs (the multiply runs in
+// registers), and its last inverse pass's outputs are the points it stores, so
+// of the 2 x passes LDS round trips only 2 x (passes - 1) remain.  R4 = 1:
+// three passes per direction.
+template <int R1, int R2, int R3, int R4, bool PAD>
+__device__ __forceinline__ void sk_filter(float2* buf, const float2* __restrict__ tw,
+                                          const float* __restrict__ H,
+                                          float2 (&x)[R1 * R2 * R3 * R4 / 256]) {
+  constexpr int N = R1 * R2 * R3 * R4;
+  constexpr int RL = R4 > 1 ? R4 : R3;      // last forward radix = first inverse radix
+  constexpr int NSL = N / RL;               // its stride in both directions
+  constexpr int B1 = sk_bpt<N, R1>(), BL = sk_bpt<N, RL>();
+  static_assert(B1 * R1 * 256 == N, "the first pass's inputs are the loaded points");
+  {
+    float2 v[B1][R1];
+#pragma unroll
+    for (int q = 0; q < B1; ++q)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[q][r] = x[q + B1 * r];
+    sk_dft<R1, B1, N, 1>(v, -1.f, tw);
+    sk_write<R1, B1, PAD, N, 1>(buf, v);
+    __syncthreads();
+  }
+  sk_pass<R2, sk_bpt<N, R2>(), PAD, N, R1>(buf, -1.f, tw);
+  if constexpr (R4 > 1) sk_pass<R3, sk_bpt<N, R3>(), PAD, N, R1 * R2>(buf, -1.f, tw);
+  {
+    float2 v[BL][RL];
+    sk_read<RL, BL, PAD, N>(buf, v);
+    sk_dft<RL, BL, N, NSL>(v, -1.f, tw);
+#pragma unroll
+    for (int q = 0; q < BL; ++q) {
+      if (sk_active<N, RL, BL>(q)) {
+        const int j = threadIdx.x + q * 256;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const float h = H[j + r * NSL];
+          v[q][r] = make_float2(v[q][r].x * h, v[q][r].y * h);
+        }
+      }
+    }
+    sk_dft<RL, BL, N, 1>(v, 1.f, tw);
+    __syncthreads();  // every thread has read the last forward pass's inputs
+    sk_write<RL, BL, PAD, N, 1>(buf, v);
+    __syncthreads();
+  }
+  if constexpr (R4 > 1) {
+    sk_pass<R3, sk_bpt<N, R3>(), PAD, N, R4>(buf, 1.f, tw);
+    sk_pass<R2, sk_bpt<N, R2>(), PAD, N, R4 * R3>(buf, 1.f, tw);
+  } else {
+    sk_pass<R2, sk_bpt<N, R2>(), PAD, N, R3>(buf, 1.f, tw);
+  }
+  {
+    float2 v[B1][R1];
+    sk_read<R1, B1, PAD, N>(buf, v);
+    sk_dft<R1, B1, N, N / R1>(v, 1.f, tw);
+#pragma unroll
+    for (int q = 0; q < B1; ++q)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) x[q + B1 * r] = v[q][r];
+  }
+}
+
+// Radix plans for N = 2^M, 9 <= M <= 13 (256 threads): the first radix is
+// N / 256 (16 at M = 13, two butterflies per thread), so every thread starts
+// with the points it loaded.  The LDS is padded for M <= 12 (N + N/16 slots; at
+// M = 13 that would pass 64 KiB).
 template <int M>
 constexpr bool sk_pad() {
   return M <= 12;
 }
 
 template <int M>
-__device__ __forceinline__ void fft_sk(float2* buf, const float2* tw, float sign, const float* H) {
-  constexpr int N = 1 << M;
+__device__ __forceinline__ void filter_sk(float2* buf, const float2* tw, const float* H,
+                                          float2 (&x)[(1 << M) / 256]) {
   constexpr bool P = sk_pad<M>();
-  if constexpr (M == 9) {
-    sk_pass<8, 1, P, N, 1>(buf, sign, tw, H);
-    sk_pass<8, 1, P, N, 8>(buf, sign, tw, nullptr);
-    sk_pass<8, 1, P, N, 64>(buf, sign, tw, nullptr);
-  } else if constexpr (M == 10) {
-    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
-    sk_pass<8, 1, P, N, 16>(buf, sign, tw, nullptr);
-    sk_pass<8, 1, P, N, 128>(buf, sign, tw, nullptr);
-  } else if constexpr (M == 11) {
-    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
-    sk_pass<16, 1, P, N, 16>(buf, sign, tw, nullptr);
-    sk_pass<8, 1, P, N, 256>(buf, sign, tw, nullptr);
-  } else if constexpr (M == 12) {
-    sk_pass<16, 1, P, N, 1>(buf, sign, tw, H);
-    sk_pass<16, 1, P, N, 16>(buf, sign, tw, nullptr);
-    sk_pass<16, 1, P, N, 256>(buf, sign, tw, nullptr);
-  } else {
+  if constexpr (M == 9) sk_filter<2, 16, 16, 1, P>(buf, tw, H, x);
+  else if constexpr (M == 10) sk_filter<4, 16, 16, 1, P>(buf, tw, H, x);
+  else if constexpr (M == 11) sk_filter<8, 16, 16, 1, P>(buf, tw, H, x);
+  else if constexpr (M == 12) sk_filter<16, 16, 16, 1, P>(buf, tw, H, x);
+  else {
     static_assert(M == 13, "plans cover 2^9 .. 2^13");
-    sk_pass<16, 2, P, N, 1>(buf, sign, tw, H);
-    sk_pass<16, 2, P, N, 16>(buf, sign, tw, nullptr);
-    sk_pass<16, 2, P, N, 256>(buf, sign, tw, nullptr);
-    sk_pass<2, 16, P, N, 4096>(buf, sign, tw, nullptr);
+    sk_filter<16, 16, 16, 2, P>(buf, tw, H, x);
   }
 }
 
@@ -363,6 +448,64 @@ __device__ inline void store16(const FilterArgs& a, int view, int u, int ra, boo
   } else {
     d[(long long)ra * a.dst_pitch + u] = qa;
     if (has_b) d[(long long)(ra + 1) * a.dst_pitch + u] = qb;
+  }
+}
+
+// Output side of the filter kernel: the 16-bit quantisation constants of the
+// view and the running min/max of the values stored.
+struct StoreCtx {
+  float qlo, qinv, lo, hi;
+};
+
+__device__ __forceinline__ StoreCtx store_ctx(const FilterArgs& a, int view) {
+  StoreCtx c{0.f, 0.f, INFINITY, -INFINITY};
+  if (a.qmode == kStoreF16 || a.qmode == kStoreBF16) {
+    c.qlo = ord2f(a.qmm[2 * view]);
+    const float range = ord2f(a.qmm[2 * view + 1]) - c.qlo;
+    c.qinv = range > 0.f ? 1.0f / range : 0.f;
+  }
+  return c;
+}
+
+// Stores output sample u of rows ra (fa) and ra+1 (fb) in the mode's form and
+// layout, and tracks their min/max.
+__device__ __forceinline__ void store_pair(const FilterArgs& a, StoreCtx& c, int view, int u, int ra,
+                                           bool has_b, float fa, float fb) {
+  if (a.qmode == kStoreNone) {
+  } else if (a.qmode == kStoreF16) {
+    store16<kStoreF16>(a, view, u, ra, has_b, fa, fb, c.qlo, c.qinv);
+  } else if (a.qmode == kStoreBF16) {
+    store16<kStoreBF16>(a, view, u, ra, has_b, fa, fb, c.qlo, c.qinv);
+  } else {
+    float* d = a.dst + view * a.dst_view;
+    if (a.transpose) {
+      float* o = d + (long long)u * a.nrows + ra;
+      if (has_b && (a.nrows & 1) == 0) {
+        *reinterpret_cast<float2*>(o) = make_float2(fa, fb);  // 8-byte aligned: u*nrows + ra even
+      } else {
+        o[0] = fa;
+        if (has_b) o[1] = fb;
+      }
+    } else {
+      d[(long long)ra * a.dst_pitch + u] = fa;
+      if (has_b) d[(long long)(ra + 1) * a.dst_pitch + u] = fb;
+    }
+  }
+  c.lo = fminf(c.lo, has_b ? fminf(fa, fb) : fa);
+  c.hi = fmaxf(c.hi, has_b ? fmaxf(fa, fb) : fa);
+}
+
+// Per-view min/max: a wave-level reduction, then one atomic per wave.
+__device__ __forceinline__ void store_minmax(const FilterArgs& a, StoreCtx& c, int view) {
+  if (!a.minmax) return;
+  float lo = c.lo, hi = c.hi;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, o));
+    hi = fmaxf(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&a.minmax[2 * view], f2ord(lo));
+    atomicMax(&a.minmax[2 * view + 1], f2ord(hi));
   }
 }
 
@@ -440,6 +583,7 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
 #pragma unroll
       for (int i = 0; i < E; ++i) wa[i] = wb[i] = 1.f;
     }
+    float2 x[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       const int n = threadIdx.x + 256 * i;
@@ -452,8 +596,19 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
         va = rra;
         vb = rrb;
       }
-      buf[sk_slot<sk_pad<M>()>(n)] = make_float2(va, vb);
+      x[i] = make_float2(va, vb);
     }
+    filter_sk<M>(buf, a.tw, a.H, x);
+    // x[i] is now output point tid + 256 i: crop, scale and store it
+    const float inv = 1.0f / (float)a.N;
+    StoreCtx sc = store_ctx(a, view);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int u = (int)threadIdx.x + 256 * i - off;
+      if (u >= 0 && u < nu) store_pair(a, sc, view, u, ra, has_b, x[i].x * inv, x[i].y * inv);
+    }
+    store_minmax(a, sc, view);
+    return;
   } else {
     for (int n = threadIdx.x; n < a.N; n += blockDim.x) {
       const int p = n - off;
@@ -468,10 +623,7 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
     }
   }
   __syncthreads();
-  if constexpr (M != 0) {
-    fft_sk<M>(buf, a.tw, -1.f, nullptr);
-    fft_sk<M>(buf, a.tw, 1.f, a.H);  // the filter multiply rides on the first inverse pass
-  } else {
+  {
     fft_lds(buf, a.tw, a.N, a.logN, -1.f);
     // Filter multiply fused with the bit-reversal permutation of the inverse pass.
     for (int k = threadIdx.x; k < a.N; k += blockDim.x) {
@@ -490,69 +642,39 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
     fft_lds(buf, a.tw, a.N, a.logN, 1.f);
   }
   const float inv = 1.0f / (float)a.N;
-  float* d = a.dst + view * a.dst_view;
-  float lo = INFINITY, hi = -INFINITY;
-  float qlo = 0.f, qinv = 0.f;
-  if (a.qmode == kStoreF16 || a.qmode == kStoreBF16) {
-    qlo = ord2f(a.qmm[2 * view]);
-    const float range = ord2f(a.qmm[2 * view + 1]) - qlo;
-    qinv = range > 0.f ? 1.0f / range : 0.f;
-  }
+  StoreCtx sc = store_ctx(a, view);
   for (int u = threadIdx.x; u < nu; u += blockDim.x) {
-    const float2 c = buf[M ? sk_slot<sk_pad<M>()>(off + u) : off + u];
-    const float fa = c.x * inv, fb = c.y * inv;
-    if (a.qmode == kStoreNone) {
-    } else if (a.qmode == kStoreF16) {
-      store16<kStoreF16>(a, view, u, ra, has_b, fa, fb, qlo, qinv);
-    } else if (a.qmode == kStoreBF16) {
-      store16<kStoreBF16>(a, view, u, ra, has_b, fa, fb, qlo, qinv);
-    } else if (a.transpose) {
-      float* o = d + (long long)u * a.nrows + ra;
-      if (has_b && (a.nrows & 1) == 0) {
-        *reinterpret_cast<float2*>(o) = make_float2(fa, fb);  // 8-byte aligned: u*nrows + ra even
-      } else {
-        o[0] = fa;
-        if (has_b) o[1] = fb;
-      }
-    } else {
-      d[(long long)ra * a.dst_pitch + u] = fa;
-      if (has_b) d[(long long)rb * a.dst_pitch + u] = fb;
-    }
-    lo = fminf(lo, has_b ? fminf(fa, fb) : fa);
-    hi = fmaxf(hi, has_b ? fmaxf(fa, fb) : fa);
+    const float2 c = buf[off + u];
+    store_pair(a, sc, view, u, ra, has_b, c.x * inv, c.y * inv);
   }
-  if (a.minmax) {
-    // wave-level reduction, then one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) {
-      lo = fminf(lo, __shfl_xor(lo, o));
-      hi = fmaxf(hi, __shfl_xor(hi, o));
-    }
-    if ((threadIdx.x & 63) == 0) {
-      atomicMin(&a.minmax[2 * view], f2ord(lo));
-      atomicMax(&a.minmax[2 * view + 1], f2ord(hi));
-    }
-  }
+  store_minmax(a, sc, view);
 }
 
-// FP32 staging -> FP16/BF16 with q = (p - min) / (max - min) per view.
-template <int DT>
+// FP32 staging -> FP16/BF16 with q = (p - min) / (max - min) per view, four
+// samples per thread per step (16-byte loads, 8-byte stores) when the view
+// size and both view strides are multiples of 4 (VEC), else one.
+template <int DT, bool VEC>
 __global__ __launch_bounds__(256) void quantize_kernel(const float* __restrict__ src, long long sview,
                                                        void* dst, long long dview, long long n,
                                                        const unsigned* minmax) {
+  constexpr int Q = DT == FDK_F16 ? kStoreF16 : kStoreBF16;
   const int view = blockIdx.y;
   const float lo = ord2f(minmax[2 * view]), hi = ord2f(minmax[2 * view + 1]);
   const float range = hi - lo;
   const float inv = range > 0.f ? 1.0f / range : 0.f;
   const float* s = src + view * sview;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    const float q = (s[i] - lo) * inv;
-    if (DT == FDK_F16) {
-      static_cast<__half*>(dst)[view * dview + i] = __float2half_rn(q);
-    } else {
-      const unsigned b = __float_as_uint(q);
-      const unsigned r = (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;  // round to nearest even
-      static_cast<unsigned short*>(dst)[view * dview + i] = (unsigned short)r;
+  unsigned short* d = static_cast<unsigned short*>(dst) + view * dview;
+  const long long stride = (long long)gridDim.x * 256;
+  if (VEC) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; 4 * i < n; i += stride) {
+      const float4 p = reinterpret_cast<const float4*>(s)[i];
+      const unsigned w0 = (unsigned)quantize16<Q>(p.x, lo, inv) | ((unsigned)quantize16<Q>(p.y, lo, inv) << 16);
+      const unsigned w1 = (unsigned)quantize16<Q>(p.z, lo, inv) | ((unsigned)quantize16<Q>(p.w, lo, inv) << 16);
+      reinterpret_cast<uint2*>(d)[i] = make_uint2(w0, w1);
     }
+  } else {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += stride)
+      d[i] = quantize16<Q>(s[i], lo, inv);
   }
 }
 
@@ -737,13 +859,21 @@ int filter_views_q(FilterArgs a, int nviews, void* dev_dst, int64_t view_stride_
   a.dst_view = (long long)view;
   a.minmax = mm;
   if (int r = launch_filter(a, nviews, st)) return r;
-  const int blocks = (int)std::min<size_t>(1024, (view + 255) / 256);
-  if (dtype == FDK_F16)
-    hipLaunchKernelGGL(quantize_kernel<FDK_F16>, dim3(blocks, nviews), dim3(256), 0, st, stage,
-                       (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
-  else
-    hipLaunchKernelGGL(quantize_kernel<FDK_BF16>, dim3(blocks, nviews), dim3(256), 0, st, stage,
-                       (long long)view, dev_dst, (long long)view_stride_out, (long long)view, mm);
+  // 16-byte aligned FP32 rows and 8-byte aligned 16-bit rows: the staging
+  // batch is a pool allocation, dev_dst's alignment is checked
+  const bool vec = view % 4 == 0 && view_stride_out % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(dev_dst) % 8 == 0;
+  const size_t items = vec ? view / 4 : view;
+  const int blocks = (int)std::min<size_t>(1024, (items + 255) / 256);
+  const dim3 g(blocks, nviews);
+  const long long n = (long long)view, sv = (long long)view, dv = (long long)view_stride_out;
+  if (dtype == FDK_F16) {
+    if (vec) hipLaunchKernelGGL((quantize_kernel<FDK_F16, true>), g, dim3(256), 0, st, stage, sv, dev_dst, dv, n, mm);
+    else hipLaunchKernelGGL((quantize_kernel<FDK_F16, false>), g, dim3(256), 0, st, stage, sv, dev_dst, dv, n, mm);
+  } else {
+    if (vec) hipLaunchKernelGGL((quantize_kernel<FDK_BF16, true>), g, dim3(256), 0, st, stage, sv, dev_dst, dv, n, mm);
+    else hipLaunchKernelGGL((quantize_kernel<FDK_BF16, false>), g, dim3(256), 0, st, stage, sv, dev_dst, dv, n, mm);
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -783,8 +913,11 @@ extern "C" int fdk_filter_views(const float* dev_src, int64_t view_stride_in, vo
   const size_t view = (size_t)nu * nv;
   float* tmp = nullptr;
   unsigned* mm = nullptr;
-  FDK_HIP(hipMallocAsync((void**)&tmp, view * nviews * sizeof(float), st));
-  FDK_HIP(hipMallocAsync((void**)&mm, 2 * sizeof(unsigned) * nviews, st));
+  if (int r = fdk::stage_alloc((void**)&tmp, view * nviews * sizeof(float), st)) return r;
+  if (int r = fdk::stage_alloc((void**)&mm, 2 * sizeof(unsigned) * nviews, st)) {
+    (void)hipFreeAsync(tmp, st);
+    return r;
+  }
   int r = filter_views_q(a, nviews, dev_dst, view_stride_out, dtype, tmp, mm, st);
   std::vector<unsigned> hmm(2 * nviews);
   if (r == FDK_OK) {
