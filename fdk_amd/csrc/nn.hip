@@ -54,13 +54,18 @@ __device__ inline bf16x8 zero8() {
 }
 
 struct Conv {
-  const u16* x;      // [n][d][h][w][cin]
+  const u16* x;      // [n][d][h][w][cin], or [n][d][h][w][cin - cup] with `up`
   const u16* wt;     // [27][cout][cin] (cin == 1: [cout][32] taps, zero padded)
   const float* bias; // [cout] or null
   const float* resid;// [n][d][h][w][cout] FP32 added to the output, or null
   void* y;           // [n][d][h][w][cout], BF16 or FP32
   int n, d, h, w, cin, cout;
   int relu, out_f32;
+  // Decoder form (conv3d_fwd4_kernel only): the input is the channel
+  // concatenation [nearest 2x upsampling of up ([n][d/2][h/2][w/2][cup]), x],
+  // read from the two tensors without forming it.  up == null: plain input.
+  const u16* up = nullptr;
+  int cup = 0;
 };
 
 template <int NT>
@@ -370,39 +375,33 @@ __device__ __forceinline__ Fwd4Tile fwd4_tile(const Conv& a, int tile, int tiles
 // is halo row s / 136 (plane-major), x s % 136 / 4 and 8-channel group s % 4.
 // Offsets are signed and relative to the tile's first voxel (z0, y0, x0), which
 // is always inside the volume; slots outside it are never addressed.  Interior
-// tiles (uniform) skip the range checks.
+// tiles (uniform) skip the range checks.  A chunk of a decoder input's
+// upsampled half (uniform) reads the coarse tensor `up`: full-resolution voxel
+// z0 - 1 + hz maps to coarse (z0 - 1 + hz) >> 1 = z0 / 2 + ((hz - 1) >> 1) (z0,
+// y0 and x0 are even), the same offset formula with every step halved.
 __device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a, const Fwd4Tile& f,
                                            int c0) {
   const int t = threadIdx.x;
-  const u16* org =
-      a.x + ((((long long)f.n * a.d + f.z0) * a.h + f.y0) * a.w + f.x0) * a.cin + c0;
-  const int plane = a.h * a.w * a.cin, rowe = a.w * a.cin;
+  const bool coarse = a.up != nullptr && c0 < a.cup;
+  const int sh = coarse ? 1 : 0;
+  const int cc = coarse ? a.cup : a.cin - a.cup;  // channels per voxel of the source
+  const int hh = a.h >> sh, ww = a.w >> sh;
+  const u16* org = (coarse ? a.up : a.x) +
+                   ((((long long)f.n * (a.d >> sh) + (f.z0 >> sh)) * hh + (f.y0 >> sh)) * ww +
+                    (f.x0 >> sh)) * cc + (coarse ? c0 : c0 - a.cup);
+  const int plane = hh * ww * cc, rowe = ww * cc;
   int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
-  if (f.interior) {
-#pragma unroll
-    for (int j = 0; j < kHaloPer; ++j) {
-      const int hz = row / kHaloRows, hy = row - hz * kHaloRows;
-      const int off = (hz - 1) * plane + (hy - 1) * rowe + ((w >> 2) - 1) * a.cin + 8 * (w & 3);
-      if (j + 1 < kHaloPer || t + 256 * j < kHaloChunks)
-        pre[j] = *reinterpret_cast<const uint4*>(org + off);
-      w += 256 - kHaloRowSlots;  // 256 = one row (136 slots) + 120
-      row += 1;
-      if (w >= kHaloRowSlots) {
-        w -= kHaloRowSlots;
-        row += 1;
-      }
-    }
-    return;
-  }
 #pragma unroll
   for (int j = 0; j < kHaloPer; ++j) {
     const int hz = row / kHaloRows, hy = row - hz * kHaloRows, hx = w >> 2;
-    const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
-    const bool ok = (j + 1 < kHaloPer || t + 256 * j < kHaloChunks) && zz >= 0 && zz < a.d &&
-                    yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
-    const int off = (hz - 1) * plane + (hy - 1) * rowe + (hx - 1) * a.cin + 8 * (w & 3);
+    bool ok = j + 1 < kHaloPer || t + 256 * j < kHaloChunks;
+    if (!f.interior) {
+      const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
+      ok = ok && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    }
+    const int off = ((hz - 1) >> sh) * plane + ((hy - 1) >> sh) * rowe + ((hx - 1) >> sh) * cc + 8 * (w & 3);
     pre[j] = ok ? *reinterpret_cast<const uint4*>(org + off) : make_uint4(0u, 0u, 0u, 0u);
-    w += 256 - kHaloRowSlots;
+    w += 256 - kHaloRowSlots;  // 256 = one row (136 slots) + 120
     row += 1;
     if (w >= kHaloRowSlots) {
       w -= kHaloRowSlots;
@@ -1268,6 +1267,29 @@ static int conv_kernel_choice() {
 
 static bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
+// Shapes and alignments conv3d_fwd4_kernel takes (cout 32 or 64, whole
+// 32-channel chunks, even depth, 16-byte vectors, 32-bit staging offsets).
+static bool lean_ok(const Conv& a) {
+  const int cx = a.cin - a.cup;
+  return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && (a.cout == 32 || a.cout == 64) &&
+         aligned16(a.x) && aligned16(a.wt) && aligned16(a.y) && (!a.bias || aligned16(a.bias)) &&
+         (!a.resid || aligned16(a.resid)) && (!a.up || aligned16(a.up)) &&
+         ((4LL * a.h * a.w + 64) * std::max(cx, a.cup) < (1LL << 31)) &&
+         27LL * a.cout * a.cin * 2 < (1LL << 31);
+}
+
+static int launch_fwd4(const Conv& a, hipStream_t st) {
+  const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / 2);
+  const int tpb = 2;
+  const dim3 g4((ntiles + tpb - 1) / tpb);
+  if (a.cout == 32)
+    hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+  else
+    hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
 extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid,
                                  void* y, int n, int d, int h, int wd, int cin, int cout, int relu,
                                  int out_f32, void* stream) {
@@ -1283,21 +1305,7 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   dim3 grid(wd / 32, h / 4, n * d);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ck = conv_kernel_choice();
-  const bool lean = ck >= 4 && cin % 32 == 0 && d % 2 == 0 && (cout == 32 || cout == 64) &&
-                    aligned16(x) && aligned16(w) && aligned16(y) && (!bias || aligned16(bias)) &&
-                    (!resid || aligned16(resid)) &&
-                    ((4LL * h * wd + 64) * cin < (1LL << 31)) && 27LL * cout * cin * 2 < (1LL << 31);
-  if (lean) {
-    const int tiles_x = wd / 32, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
-    const int tpb = 2;
-    const dim3 g4((ntiles + tpb - 1) / tpb);
-    if (cout == 32)
-      hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
-    else
-      hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
-    FDK_HIP(hipGetLastError());
-    return FDK_OK;
-  }
+  if (ck >= 4 && lean_ok(a)) return launch_fwd4(a, st);
   const bool staged = cin % 32 == 0 && d % 2 == 0 && cout <= 128 && ck >= 3;
   const bool tiled = !staged && cin % 32 == 0 && h % 8 == 0 && cout <= 128 && ck >= 2;
   dim3 grid2(wd / 32, h / 8, n * d);
@@ -1321,6 +1329,25 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
 #undef FDK_CONV_CASE
   FDK_HIP(hipGetLastError());
   return FDK_OK;
+}
+
+extern "C" int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, const void* w,
+                                     const float* bias, const float* resid, void* y, int n, int d,
+                                     int h, int wd, int cin, int cout, int relu, int out_f32,
+                                     void* stream) {
+  FDK_REQUIRE(up && x && w && y, "NULL pointer");
+  FDK_REQUIRE(n > 0 && d > 0 && h > 0 && wd > 0, "bad tensor size");
+  FDK_REQUIRE(wd % 32 == 0 && h % 4 == 0 && d % 2 == 0,
+              "conv3d_fwd_cat needs W %% 32 == 0, H %% 4 == 0 and even D (got %d, %d, %d)", wd, h, d);
+  FDK_REQUIRE(cup > 0 && cup < cin && cup % 32 == 0 && cin % 32 == 0,
+              "conv3d_fwd_cat needs 0 < cup < cin, both multiples of 32 (got %d, %d)", cup, cin);
+  FDK_REQUIRE(relu >= 0 && relu <= 2, "activation mode %d not in {0, 1, 2}", relu);
+  Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
+         cin, cout, relu, out_f32, static_cast<const u16*>(up), cup};
+  if (!lean_ok(a))
+    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_fwd_cat: cout %d (32 or 64), 16-byte aligned "
+                     "tensors and 32-bit halo offsets required", cout);
+  return launch_fwd4(a, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h,

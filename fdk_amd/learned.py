@@ -32,6 +32,7 @@ I0 = 1.0e5        # photons per detector pixel without attenuation
 MU = 0.02         # attenuation per mm for unit phantom density (about water)
 PATCH = 128
 OVERLAP = 16
+SLAB = 528  # z-slab depth of the full-resolution network's inference
 
 
 def geometry(n: int = 1024, views: int = 1024) -> Parameter:
@@ -234,14 +235,17 @@ def infer_tiled(net: UNet3D, vol, out=None, batch: int = 4):
 def infer_slabs(net: UNet3D, vol, out=None, slab: int | None = None, overlap: int = OVERLAP):
     """Apply the network to a (D, H, W) FP32 device volume in z-slabs of whole
     x-y slices (slab deep, neighbouring slabs overlapping by ``overlap`` and
-    split at the midpoint), batch 1: 1.125x redundant work at 1024^3 with
-    128-deep slabs against 1.43x for 128^3 tiles.  slab=None: 128 for the
-    full-resolution network, the whole volume in one pass (no redundant work)
-    for the space-to-depth networks.  D >= slab; H, W as UNet3D requires."""
+    split at the midpoint), batch 1.  slab=None: SLAB (528) for the
+    full-resolution network -- two slabs at 1024^3, 1.03x redundant work
+    (1.125x with 128-deep slabs, 1.43x with 128^3 tiles), its activations
+    peaking near 144 bytes per slab voxel (80 GB at 1024^2 x 528) -- and the
+    whole volume in one pass (no redundant work) for the space-to-depth
+    networks.  A volume no deeper than the slab runs in one pass; H, W as
+    UNet3D requires."""
     import torch
     D = vol.shape[0]
     if slab is None:
-        slab = D if net.s2d > 1 else PATCH
+        slab = D if net.s2d > 1 else SLAB
     out = torch.empty_like(vol) if out is None else out
     if slab >= D:
         net.forward(vol.unsqueeze(0), out=out.unsqueeze(0))

@@ -46,6 +46,7 @@ def _sig():
     L = lib()
     vp, i, i64, f = C.c_void_p, C.c_int, C.c_int64, C.c_float
     L.fdk_nn_conv3d_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
+    L.fdk_nn_conv3d_fwd_cat.argtypes = [vp, i, vp, vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
     L.fdk_nn_conv3d_wgrad.argtypes = [vp, vp, vp, i, i, i, i, i, i, vp]
     L.fdk_nn_flip_weights.argtypes = [vp, vp, i, i, vp]
     L.fdk_nn_bias_grad.argtypes = [vp, vp, i64, i, vp]
@@ -83,6 +84,15 @@ def conv3d(x, w, bias, y, shape, cin, cout, relu=True, out_f32=False, resid=None
     n, d, h, wd = shape
     _check(L().fdk_nn_conv3d_fwd(_p(x), _p(w), _p(bias), _p(resid), _p(y), n, d, h, wd, cin, cout,
                                  int(relu), int(out_f32), _st()), "conv3d")
+    return y
+
+
+def conv3d_cat(up, cup, x, w, bias, y, shape, cin, cout, relu=True):
+    """conv3d on the concatenation [nearest 2x upsampling of up (cup channels), x]
+    without forming it; shape is the full-resolution (n, d, h, w)."""
+    n, d, h, wd = shape
+    _check(L().fdk_nn_conv3d_fwd_cat(_p(up), int(cup), _p(x), _p(w), _p(bias), None, _p(y), n, d,
+                                     h, wd, cin, cout, int(relu), 0, _st()), "conv3d_cat")
     return y
 
 
@@ -215,6 +225,16 @@ class Conv3:
                         device=x.device)
         return conv3d(x, self.wbf, self.b, y, shape, self.cin, self.cout, self.relu, out_f32)
 
+    def cat_ok(self, cup):
+        """Whether forward_cat serves this layer (the fused decoder-input kernel)."""
+        return self.cout in (32, 64) and self.cin % 32 == 0 and cup % 32 == 0 and 0 < cup < self.cin
+
+    def forward_cat(self, up, cup, x, shape):
+        """forward() of [upsample2(up), x] (channels concatenated) without forming it."""
+        torch = _torch()
+        y = torch.empty(shape + (self.cout,), dtype=I16, device=x.device)
+        return conv3d_cat(up, cup, x, self.wbf, self.b, y, shape, self.cin, self.cout, self.relu)
+
     def backward(self, dy, x, y, shape, need_dx=True):
         """dy: gradient at the (post-ReLU) output y; x: the layer input."""
         torch = _torch()
@@ -331,25 +351,36 @@ class UNet3D:
         bb = self.bb.forward(ba, s3_)
         hold(ba=ba)
         del ba
-        cat2 = bf(s2_, c3 + c2)
-        upsample(bb, cat2, s3_, c3, c3 + c2, 0)
-        channel_copy(s2, c2, 0, cat2, c3 + c2, c3, c2, int(np.prod(s2_)))
-        hold(bb=bb, s2=s2)
-        del bb, s2
-        d2a = self.d2a.forward(cat2, s2_)
-        hold(cat2=cat2)
-        del cat2
+        # Decoder inputs [upsample2(coarse), skip]: formed only when backward()
+        # needs them (keep) or the fused kernel does not serve the layer.
+        fuse = not keep and self.d2a.cat_ok(c3) and self.d1a.cat_ok(c2)
+        if fuse:
+            d2a = self.d2a.forward_cat(bb, c3, s2, s2_)
+            del bb, s2
+        else:
+            cat2 = bf(s2_, c3 + c2)
+            upsample(bb, cat2, s3_, c3, c3 + c2, 0)
+            channel_copy(s2, c2, 0, cat2, c3 + c2, c3, c2, int(np.prod(s2_)))
+            hold(bb=bb, s2=s2)
+            del bb, s2
+            d2a = self.d2a.forward(cat2, s2_)
+            hold(cat2=cat2)
+            del cat2
         d2 = self.d2b.forward(d2a, s2_)
         hold(d2a=d2a)
         del d2a
-        cat1 = bf(s1_, c2 + c1)
-        upsample(d2, cat1, s2_, c2, c2 + c1, 0)
-        channel_copy(s1, c1, 0, cat1, c2 + c1, c2, c1, int(np.prod(s1_)))
-        hold(d2=d2, s1=s1)
-        del d2, s1
-        d1a = self.d1a.forward(cat1, s1_)
-        hold(cat1=cat1)
-        del cat1
+        if fuse:
+            d1a = self.d1a.forward_cat(d2, c2, s1, s1_)
+            del d2, s1
+        else:
+            cat1 = bf(s1_, c2 + c1)
+            upsample(d2, cat1, s2_, c2, c2 + c1, 0)
+            channel_copy(s1, c1, 0, cat1, c2 + c1, c2, c1, int(np.prod(s1_)))
+            hold(d2=d2, s1=s1)
+            del d2, s1
+            d1a = self.d1a.forward(cat1, s1_)
+            hold(cat1=cat1)
+            del cat1
         d1 = self.d1b.forward(d1a, s1_)
         hold(d1a=d1a)
         del d1a

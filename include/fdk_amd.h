@@ -312,6 +312,15 @@ int fdk_phantom_project(const fdk_ellipsoid* host_ell, int n, const fdk_geometry
 int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const float* resid, void* y,
                       int n, int d, int h, int w_, int cin, int cout, int relu, int out_f32,
                       void* stream);
+/* The same convolution on a decoder input that is the channel concatenation
+ * [nearest 2x upsampling of up, x] (up: [n][d/2][h/2][w/2][cup], x:
+ * [n][d][h][w][cin - cup]), read from the two tensors without forming the
+ * concatenation.  Bit-identical to fdk_nn_conv3d_fwd on the formed input.
+ * Needs cout 32 or 64, cup and cin multiples of 32, even d, 16-byte aligned
+ * tensors; FDK_ERR_UNSUPPORTED otherwise. */
+int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, const void* w, const float* bias,
+                          const float* resid, void* y, int n, int d, int h, int w_, int cin,
+                          int cout, int relu, int out_f32, void* stream);
 /* dW[27][cout][cin] (FP32, accumulated) += sum_v dY[v][co] X[v + tap][ci];
  * cin == 1 accumulates into [cout][32]. */
 int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h, int w_,

@@ -90,6 +90,51 @@ def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
         assert rel(y32.cpu().numpy(), R.conv3d(x, w) + res) < 1e-5
 
 
+@pytest.mark.parametrize("cup,cskip,cout,shape", [(64, 32, 32, (2, 8, 12, 96)),
+                                                  (128, 64, 64, (1, 8, 12, 96)),
+                                                  (64, 32, 32, (1, 2, 4, 32))])
+def test_conv3d_decoder_input_fused(torch_cuda, cup, cskip, cout, shape):
+    """conv3d of [nearest 2x upsampling of a coarse tensor, skip] read from the two
+    tensors (fdk_nn_conv3d_fwd_cat) is bit-identical to upsample + channel copy +
+    conv3d, on interior and border tiles of both batch items, and matches the
+    oracle on the formed input."""
+    torch = torch_cuda
+    rng = np.random.default_rng(cup + cout + shape[0])
+    n, d, h, w = shape
+    cin = cup + cskip
+    small = (n, d // 2, h // 2, w // 2)
+    up = R.bf16_round(rng.standard_normal(small + (cup,)))
+    skip = R.bf16_round(rng.standard_normal(shape + (cskip,)))
+    wt = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    up_d, skip_d, w_d, b_d = bf(torch, up), bf(torch, skip), bf(torch, wt), torch.as_tensor(b).cuda()
+    cat = torch.empty(shape + (cin,), dtype=torch.int16, device="cuda")
+    U.upsample(up_d, cat, small, cup, cin, 0)
+    U.channel_copy(skip_d, cskip, 0, cat, cin, cup, cskip, n * d * h * w)
+    ref = torch.empty(shape + (cout,), dtype=torch.int16, device="cuda")
+    U.conv3d(cat, w_d, b_d, ref, shape, cin, cout, relu=True)
+    got = torch.empty_like(ref)
+    U.conv3d_cat(up_d, cup, skip_d, w_d, b_d, got, shape, cin, cout, relu=True)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    full = np.concatenate([up.repeat(2, 1).repeat(2, 2).repeat(2, 3), skip], axis=-1)
+    assert rel(from_bf(got), np.maximum(R.conv3d(full, wt, b), 0)) < 4e-3
+
+
+def test_unet_inference_fused_decoder_matches_formed(torch_cuda):
+    """Inference (decoder inputs read from the coarse and skip tensors) gives the
+    same output as the forward pass that forms them for backward()."""
+    torch = torch_cuda
+    net = U.UNet3D(base=32, seed=8)
+    with torch.no_grad():
+        net.hw.mul_(300.0)
+    x = torch.as_tensor(np.random.default_rng(9).standard_normal((1, 8, 16, 128)).astype(np.float32)).cuda()
+    fused = net.forward(x).clone()
+    formed = net.forward(x, keep=True)
+    torch.cuda.synchronize()
+    assert torch.equal(fused, formed)
+
+
 def test_conv3d_forward_single_input_channel(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(5)
