@@ -483,12 +483,12 @@ def learned_sparse_view():
     The first network present is the headline (speedup, RMSE, SSIM at top
     level)."""
     from fdk_amd import learned as Lr
-    from fdk_amd.unet import UNet3D
+    from fdk_amd.unet import NativeUNet
     nets = {}
     for name, fname in LEARNED_WEIGHTS:
         w = os.path.join(HERE, "fdk_amd", "weights", fname)
-        if os.path.exists(w):
-            nets[name] = (UNet3D.from_file(w), "fdk_amd/weights/" + fname)
+        if os.path.exists(w):  # the library's C++ runner (fdk_unet_load / fdk_unet_infer)
+            nets[name] = (NativeUNet(w), "fdk_amd/weights/" + fname)
     if not nets:
         return None
     p = Lr.geometry(1024, 1024)

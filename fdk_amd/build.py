@@ -28,7 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
-SOURCES = ["capi.cpp", "geometry.cpp", "bp.hip", "filter.hip", "fp.hip", "nn.hip", "pipeline.cpp"]
+SOURCES = ["capi.cpp", "geometry.cpp", "bp.hip", "filter.hip", "fp.hip", "nn.hip", "pipeline.cpp",
+           "unet.cpp"]
 
 
 def _run(cmd):
@@ -55,7 +56,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     drv_src = os.path.join(CSRC, "drivers.cpp")
     if os.path.exists(drv_src):

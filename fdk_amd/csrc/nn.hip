@@ -589,6 +589,84 @@ __global__ __launch_bounds__(256) void conv3d_fwd_c1_kernel(Conv a) {
   conv_epilogue<NT>(a, acc, nz * a.h + y, x0, lane);
 }
 
+// cin == 1, row form: one wave per (n, z, y) row, stepping along x 32 voxels
+// (two 16-voxel tiles) at a time, so a launch has h / 4 x n d blocks instead of
+// one per 32 x 4 voxels.  The MFMA takes the weights as A and the voxels' taps
+// as B, so lane (m, kq) ends with channels 4 kq .. 4 kq + 3 of voxel m: one
+// 8-byte BF16 (16-byte FP32) store per tile and 16 channels.  The row offsets
+// and z / y validity of a lane's 8 taps are fixed for the whole row; x is
+// checked only in the first and last steps.
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd_c1r_kernel(Conv a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const int y = blockIdx.x * 4 + wave;
+  const int nz = blockIdx.y;
+  const int z = nz % a.d, n = nz / a.d;
+  bf16x8 wf[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) wf[nt] = load8(a.wt + (nt * 16 + m) * 32 + kq * 8);
+  long long rowb[8];  // element offset of tap j's input row, + dx
+  int dxs[8];
+  bool tap_ok[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tap = kq * 8 + j;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int zz = z + dz, yy = y + dy;
+    tap_ok[j] = tap < 27 && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h;
+    dxs[j] = dx;
+    rowb[j] = (((long long)n * a.d + zz) * a.h + yy) * a.w + dx;
+  }
+  float4 bias[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt)
+    bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + nt * 16 + 4 * kq)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+  const long long out_row = ((long long)nz * a.h + y) * a.w;
+  for (int x0 = 0; x0 < a.w; x0 += 32) {
+    const bool inner = x0 >= 1 && x0 + 33 <= a.w;  // uniform
+    bf16x8 vf[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const int x = x0 + mt * 16 + m;
+      u16 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool ok = tap_ok[j] && (inner || ((unsigned)(x + dxs[j]) < (unsigned)a.w));
+        v[j] = ok ? a.x[rowb[j] + x] : (u16)0;
+      }
+      const uint4 packed = make_uint4(v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16),
+                                      v[4] | ((unsigned)v[5] << 16), v[6] | ((unsigned)v[7] << 16));
+      vf[mt] = __builtin_bit_cast(bf16x8, packed);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const long long vox = out_row + x0 + mt * 16 + m;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const f32x4 acc = mfma(wf[nt], vf[mt], f32x4{0.f, 0.f, 0.f, 0.f});
+        const long long o = vox * a.cout + nt * 16 + 4 * kq;
+        float r[4] = {acc[0] + bias[nt].x, acc[1] + bias[nt].y, acc[2] + bias[nt].z,
+                      acc[3] + bias[nt].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = activate(r[i], a.relu);
+        if (a.resid) {
+          const float4 q = *reinterpret_cast<const float4*>(a.resid + o);
+          r[0] += q.x;
+          r[1] += q.y;
+          r[2] += q.z;
+          r[3] += q.w;
+        }
+        if (a.out_f32)
+          *reinterpret_cast<float4*>(static_cast<float*>(a.y) + o) = make_float4(r[0], r[1], r[2], r[3]);
+        else
+          *reinterpret_cast<uint2*>(static_cast<u16*>(a.y) + o) = pack4_bf16(r[0], r[1], r[2], r[3]);
+      }
+    }
+  }
+}
+
 // Weight gradient dW[tap][co][ci] += sum over voxels dY[v][co] X[v + tap][ci]
 // (K = voxels).  A block owns one tap and a run of `rows` output rows (n, z, y);
 // per row it stages dY and the tap-shifted X row in LDS transposed to [c][x] so
@@ -1306,6 +1384,17 @@ extern "C" int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ck = conv_kernel_choice();
   if (ck >= 4 && lean_ok(a)) return launch_fwd4(a, st);
+  if (ck >= 4 && cin == 1 && aligned16(y) && (!bias || aligned16(bias)) &&
+      (!resid || aligned16(resid))) {
+    const dim3 gr(h / 4, n * d);
+    switch (cout / 16) {
+      case 2: hipLaunchKernelGGL(conv3d_fwd_c1r_kernel<2>, gr, dim3(256), 0, st, a); break;
+      case 4: hipLaunchKernelGGL(conv3d_fwd_c1r_kernel<4>, gr, dim3(256), 0, st, a); break;
+      default: return fdk::fail(FDK_ERR_UNSUPPORTED, "cin 1: cout %d not 32 or 64", cout);
+    }
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
   const bool staged = cin % 32 == 0 && d % 2 == 0 && cout <= 128 && ck >= 3;
   const bool tiled = !staged && cin % 32 == 0 && h % 8 == 0 && cout <= 128 && ck >= 2;
   dim3 grid2(wd / 32, h / 8, n * d);

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import (MAX_PROJ, PAD_MEAN4, BackProjectionFDK, Parameter, filter_views,
                phantom_project, phantom_rasterize, random_ellipsoids)
-from .unet import UNet3D, box_copy, mse, poisson_noise
+from .unet import NativeUNet, UNet3D, box_copy, mse, poisson_noise
 
 I0 = 1.0e5        # photons per detector pixel without attenuation
 MU = 0.02         # attenuation per mm for unit phantom density (about water)
@@ -241,8 +241,11 @@ def infer_slabs(net: UNet3D, vol, out=None, slab: int | None = None, overlap: in
     peaking near 144 bytes per slab voxel (80 GB at 1024^2 x 528) -- and the
     whole volume in one pass (no redundant work) for the space-to-depth
     networks.  A volume no deeper than the slab runs in one pass; H, W as
-    UNet3D requires."""
+    UNet3D requires.  A NativeUNet runs the same schedule in the library
+    (fdk_unet_infer)."""
     import torch
+    if isinstance(net, NativeUNet):  # the library's C++ runner, same schedule
+        return net.infer(vol, out, 0 if slab is None else slab, overlap)
     D = vol.shape[0]
     if slab is None:
         slab = D if net.s2d > 1 else SLAB
@@ -284,7 +287,7 @@ def ssim_slices(a, b) -> float:
 def compare_many(nets: dict, scanner: Scanner, seed: int, sparse_step: int = 4, log=None):
     """Whole-volume comparison on one held-out phantom: plain FDK from all views
     vs FDK from every sparse_step-th view, with and without each network of
-    `nets` (name -> UNet3D).  Times are GPU wall times with the noisy projections
+    `nets` (name -> UNet3D or NativeUNet).  Times are GPU wall times with the noisy projections
     already in HBM; each network is warmed once on the same volume first."""
     import torch
     p = scanner.p

@@ -21,7 +21,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _check, lib
+from . import FdkError, _check, lib
 
 I16 = None  # torch.int16, set lazily (BF16 bit patterns)
 
@@ -66,6 +66,13 @@ def _sig():
     L.fdk_nn_box_copy.argtypes = [vp, vp, vp, vp, vp, vp, vp, f, vp]
     L.fdk_poisson_noise.argtypes = [vp, i64, C.c_double, C.c_double, C.c_uint32, vp]
     L.fdk_nn_mfma_probe.argtypes = [vp, vp, vp, vp]
+    ip = C.POINTER(C.c_int)
+    L.fdk_unet_inspect.argtypes = [C.c_char_p, ip, ip, ip]
+    L.fdk_unet_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.fdk_unet_info.argtypes = [vp, ip, ip, ip]
+    L.fdk_unet_forward.argtypes = [vp, vp, vp, i, i, i, i, vp]
+    L.fdk_unet_infer.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
+    L.fdk_unet_destroy.argtypes = [vp]
     return L
 
 
@@ -515,3 +522,67 @@ class UNet3D:
         net = cls(base=base, seed=0, device=device, s2d=s2d, act=act)
         net.load_state_dict(d)
         return net
+
+
+# ------------------------------------------------------ native inference runner
+def inspect(path) -> tuple:
+    """(base, s2d, act) of a weight file, read by the library (no device)."""
+    b, f, a = C.c_int(), C.c_int(), C.c_int()
+    _check(L().fdk_unet_inspect(str(path).encode(), C.byref(b), C.byref(f), C.byref(a)),
+           "fdk_unet_inspect")
+    return b.value, f.value, a.value
+
+
+class NativeUNet:
+    """The library's own U-Net inference (fdk_unet_*): weights loaded from a file
+    by the C++ runner, the forward pass sequenced natively from the same kernels
+    as UNet3D (bit-identical outputs).  Bound to the current device."""
+
+    def __init__(self, path):
+        self.h = C.c_void_p()
+        _check(L().fdk_unet_load(str(path).encode(), C.byref(self.h)), "fdk_unet_load")
+        b, f, a = C.c_int(), C.c_int(), C.c_int()
+        _check(L().fdk_unet_info(self.h, C.byref(b), C.byref(f), C.byref(a)), "fdk_unet_info")
+        self.base, self.s2d, self.act = b.value, f.value, a.value
+
+    @staticmethod
+    def _volume(t, name, ndim):
+        torch = _torch()
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == ndim):
+            raise FdkError(f"NativeUNet: {name} must be a contiguous {ndim}-D float32 CUDA tensor")
+
+    def forward(self, x32, out=None):
+        """x32 (N, D, H, W) FP32 device tensor -> x + net(x)."""
+        torch = _torch()
+        out = torch.empty_like(x32) if out is None else out
+        self._volume(x32, "input", 4)
+        self._volume(out, "output", 4)
+        if out.shape != x32.shape or out.data_ptr() == x32.data_ptr():
+            raise FdkError("NativeUNet: output must match the input's shape and not alias it")
+        n, d, h, w = x32.shape
+        _check(L().fdk_unet_forward(self.h, _p(x32), _p(out), n, d, h, w, _st()), "fdk_unet_forward")
+        return out
+
+    def infer(self, vol, out=None, slab: int = 0, overlap: int = 16):
+        """(D, H, W) FP32 device volume in z-slabs (slab 0: the runner's default)."""
+        torch = _torch()
+        out = torch.empty_like(vol) if out is None else out
+        self._volume(vol, "volume", 3)
+        self._volume(out, "output", 3)
+        if out.shape != vol.shape:
+            raise FdkError("NativeUNet: output must match the volume's shape")
+        d, h, w = vol.shape
+        _check(L().fdk_unet_infer(self.h, _p(vol), _p(out), d, h, w, int(slab), int(overlap),
+                                  _st()), "fdk_unet_infer")
+        return out
+
+    def close(self):
+        if self.h:
+            L().fdk_unet_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -382,6 +382,30 @@ int fdk_poisson_noise(float* dev_proj, int64_t n, double i0, double mu, uint32_t
  * FP32 in, BF16-rounded operands); a lane-layout check. */
 int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void* stream);
 
+/* ---- Native U-Net inference (the learned sparse-view correction; no reference
+ * counterpart).  A weight file is the .npz the trainer writes (UNet3D.save in
+ * fdk_amd/unet.py: conv0..conv9 .w / .b, head.w / .b, arch.s2d, arch.act); the
+ * network runs from the fdk_nn_* kernels above, bit-identical to the Python
+ * sequencing of the same kernels.  A handle is bound to the device current at
+ * fdk_unet_load. */
+typedef struct fdk_unet fdk_unet;
+/* Architecture of a weight file (base channels, space-to-depth edge 1/2/4,
+ * activation 1 ReLU / 2 leaky ReLU) without touching a device. */
+int fdk_unet_inspect(const char* path, int* base, int* s2d, int* act);
+int fdk_unet_load(const char* path, fdk_unet** out);
+int fdk_unet_info(const fdk_unet* u, int* base, int* s2d, int* act);
+/* out = x + net(x) for FP32 device volumes [n][d][h][w]; d % 4f, h % 16f and
+ * w % 128f == 0 (f the space-to-depth edge). */
+int fdk_unet_forward(fdk_unet* u, const float* dev_x, float* dev_out, int n, int d, int h, int w,
+                     void* stream);
+/* Whole-volume inference of a [d][h][w] FP32 device volume in z-slabs of whole
+ * slices, `slab` deep (<= 0: the whole volume for space-to-depth networks, 528
+ * for full resolution), neighbours overlapping by `overlap` planes split at the
+ * midpoint; dev_out must not alias dev_vol. */
+int fdk_unet_infer(fdk_unet* u, const float* dev_vol, float* dev_out, int d, int h, int w, int slab,
+                   int overlap, void* stream);
+int fdk_unet_destroy(fdk_unet* u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,17 +135,29 @@ def test_unet_inference_fused_decoder_matches_formed(torch_cuda):
     assert torch.equal(fused, formed)
 
 
-def test_conv3d_forward_single_input_channel(torch_cuda):
+@pytest.mark.parametrize("shape,cout", [((1, 4, 8, 64), 32), ((2, 3, 8, 96), 32),
+                                         ((1, 5, 4, 128), 64)])
+def test_conv3d_forward_single_input_channel(torch_cuda, shape, cout):
+    """The network's first convolution (one input channel, the 27 taps as K):
+    border and interior x steps of the row kernel, both batch items, FP32 output
+    with a residual and BF16 output with bias and activation."""
     torch = torch_cuda
-    rng = np.random.default_rng(5)
-    shape = (1, 4, 8, 64)
+    rng = np.random.default_rng(5 + cout)
     x = R.bf16_round(rng.standard_normal(shape + (1,)))
-    w27 = R.bf16_round(rng.standard_normal((27, 32, 1)) / 5)
-    wpad = np.zeros((32, 32))
+    w27 = R.bf16_round(rng.standard_normal((27, cout, 1)) / 5)
+    wpad = np.zeros((cout, 32))
     wpad[:, :27] = w27[:, :, 0].T
-    y = torch.empty(shape + (32,), dtype=torch.float32, device="cuda")
-    U.conv3d(bf(torch, x[..., 0]), bf(torch, wpad), None, y, shape, 1, 32, relu=False, out_f32=True)
-    assert rel(y.cpu().numpy(), R.conv3d(x, w27)) < 1e-5
+    res = rng.standard_normal(shape + (cout,)).astype(np.float32)
+    y = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
+    U.conv3d(bf(torch, x[..., 0]), bf(torch, wpad), None, y, shape, 1, cout, relu=False,
+             out_f32=True, resid=torch.as_tensor(res).cuda())
+    assert rel(y.cpu().numpy(), R.conv3d(x, w27) + res) < 1e-5
+    b = rng.standard_normal(cout).astype(np.float32)
+    yb = torch.empty(shape + (cout,), dtype=torch.int16, device="cuda")
+    U.conv3d(bf(torch, x[..., 0]), bf(torch, wpad), torch.as_tensor(b).cuda(), yb, shape, 1, cout,
+             relu=2)
+    z = R.conv3d(x, w27, b)
+    assert rel(from_bf(yb), np.where(z > 0, z, 0.01 * z)) < 4e-3
 
 
 @pytest.mark.parametrize("cin,cout,W", [(32, 64, 64), (192, 64, 64), (96, 32, 128), (64, 128, 32)])
@@ -380,3 +392,37 @@ def test_unet_training_reduces_loss(torch_cuda, s2d, act):
     tail = float(np.mean(losses[-10:]))
     print("training loss:", [round(l, 1) for l in losses[::steps // 5]], "-> last ten", round(tail, 1))
     assert tail < 0.5 * losses[0]
+
+
+@pytest.mark.parametrize("fname", ["unet_sparse1024_s2d4.npz", "unet_sparse1024_s2d2.npz",
+                                   "unet_sparse1024.npz"])
+def test_native_runner_matches_python_sequencing(torch_cuda, fname):
+    """fdk_unet_load / fdk_unet_forward (weights read and the pass sequenced in
+    C++) give the same output, bit for bit, as UNet3D.from_file on the same file,
+    and so does whole-volume slab inference against learned.infer_slabs."""
+    import os
+    import fdk_amd.learned as Lr
+    torch = torch_cuda
+    path = os.path.join(os.path.dirname(U.__file__), "weights", fname)
+    net = U.UNet3D.from_file(path)
+    nat = U.NativeUNet(path)
+    assert (nat.base, nat.s2d, nat.act) == (32, net.s2d, net.act)
+    f = net.s2d
+    shape = (2, 4 * f, 16 * f, 128 * f)
+    x = torch.as_tensor(np.random.default_rng(11).standard_normal(shape).astype(np.float32)).cuda()
+    want = net.forward(x)
+    got = nat.forward(x)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    # slabs: 8 f deep with 4 f of overlap over a 28 f-deep volume (6 slabs, the
+    # last flush with the end), against the Python slab loop
+    vol = torch.as_tensor(np.random.default_rng(12).standard_normal((28 * f, 16 * f, 128 * f))
+                          .astype(np.float32)).cuda()
+    want = Lr.infer_slabs(net, vol, slab=8 * f, overlap=4 * f)
+    got = nat.infer(vol, slab=8 * f, overlap=4 * f)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    whole = nat.infer(vol, slab=28 * f)
+    torch.cuda.synchronize()
+    assert torch.equal(whole, net.forward(vol.unsqueeze(0))[0])
+    nat.close()
