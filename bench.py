@@ -556,7 +556,9 @@ def stream_leg(views: int, pool: int, source: str, warmup: int, log_steps: bool 
     log(f"generated the view pool in {time.perf_counter() - t0:.1f} s")
     z0 = n // 2 - nzs // 2
     top, bottom = (0, 0, z0, n, n, nzs // 2), (0, 0, n // 2, n, n, nzs // 2)
-    pipe = F.Pipeline(p, top, bottom, store_dtype=store_dt, slots=2)
+    # three batch slots: the copy of batch k+1 may start while batch k-1 is
+    # still being backprojected (2 GB of raw FP32 views per batch at 4096^2)
+    pipe = F.Pipeline(p, top, bottom, store_dtype=store_dt, slots=3)
 
     def push(v0):
         s0 = v0 % pool
