@@ -66,6 +66,8 @@ struct Conv {
   // read from the two tensors without forming it.  up == null: plain input.
   const u16* up = nullptr;
   int cup = 0;
+  // conv3d_fwd4_kernel computes output channels [co0, co0 + 16 NT) of the cout.
+  int co0 = 0;
 };
 
 template <int NT>
@@ -324,7 +326,8 @@ __global__ __launch_bounds__(256) void conv3d_fwd3_kernel(Conv a) {
 }
 
 // Lean form of the LDS-halo kernel for cout = 32 (NT = 2, PF) and 64 (NT = 4,
-// no PF: its registers cannot also hold the next halo), on the same
+// no PF: its registers cannot also hold the next halo; cout 128 and 192 run
+// as 64-channel passes of it, output channels co0 .. co0 + 63), on the same
 // tiles (2 planes x 4 rows x 32 voxels per block, 1 plane x 2 rows per wave)
 // and halo layout as conv3d_fwd3_kernel:
 //  * the 27 taps are unrolled, so every A-fragment read is one ds_read_b128
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const uint4* hb = halo + ((pz * kHaloRows + py) * 4 + kq) * kHaloX + m;
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), fdk::kRawRsrcWord3);
-  const unsigned lane_off = (unsigned)(kq * 8 + m * a.cin) * 2u;
+  const unsigned lane_off = (unsigned)(kq * 8 + (a.co0 + m) * a.cin) * 2u;
   const unsigned tap_bytes = (unsigned)(a.cout * a.cin) * 2u, nt_bytes = (unsigned)(16 * a.cin) * 2u;
   const int nchunks = a.cin / 32;
   const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     float4 bias[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + nt * 16 + 4 * kq)
+      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + a.co0 + nt * 16 + 4 * kq)
                         : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -527,7 +530,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
           ((long long)zo * a.h + f.y0 + py + (mt >> 1)) * a.w + f.x0 + 16 * (mt & 1) + m;
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
-        const long long o = vox * a.cout + nt * 16 + 4 * kq;
+        const long long o = vox * a.cout + a.co0 + nt * 16 + 4 * kq;
         float v[4] = {acc[mt][nt][0] + bias[nt].x, acc[mt][nt][1] + bias[nt].y,
                       acc[mt][nt][2] + bias[nt].z, acc[mt][nt][3] + bias[nt].w};
 #pragma unroll
@@ -1345,11 +1348,11 @@ static int conv_kernel_choice() {
 
 static bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
-// Shapes and alignments conv3d_fwd4_kernel takes (cout 32 or 64, whole
+// Shapes and alignments conv3d_fwd4_kernel takes (cout 32 or a multiple of 64, whole
 // 32-channel chunks, even depth, 16-byte vectors, 32-bit staging offsets).
 static bool lean_ok(const Conv& a) {
   const int cx = a.cin - a.cup;
-  return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && (a.cout == 32 || a.cout == 64) &&
+  return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && (a.cout == 32 || a.cout % 64 == 0) &&
          aligned16(a.x) && aligned16(a.wt) && aligned16(a.y) && (!a.bias || aligned16(a.bias)) &&
          (!a.resid || aligned16(a.resid)) && (!a.up || aligned16(a.up)) &&
          ((4LL * a.h * a.w + 64) * std::max(cx, a.cup) < (1LL << 31)) &&
@@ -1360,10 +1363,15 @@ static int launch_fwd4(const Conv& a, hipStream_t st) {
   const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / 2);
   const int tpb = 2;
   const dim3 g4((ntiles + tpb - 1) / tpb);
-  if (a.cout == 32)
+  if (a.cout == 32) {
     hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
-  else
-    hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
+  } else {
+    // 64 output channels per pass; cout 128 and 192 re-stage the halo per pass,
+    // which costs less than the wider accumulator set would in registers
+    Conv p = a;
+    for (p.co0 = 0; p.co0 < a.cout; p.co0 += 64)
+      hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, p, tiles_x, tiles_y, ntiles, tpb);
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -1434,7 +1442,7 @@ extern "C" int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, con
   Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
          cin, cout, relu, out_f32, static_cast<const u16*>(up), cup};
   if (!lean_ok(a))
-    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_fwd_cat: cout %d (32 or 64), 16-byte aligned "
+    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_fwd_cat: cout %d (32 or a multiple of 64), 16-byte aligned "
                      "tensors and 32-bit halo offsets required", cout);
   return launch_fwd4(a, static_cast<hipStream_t>(stream));
 }

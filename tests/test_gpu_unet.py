@@ -65,13 +65,13 @@ def test_conv3d_forward(torch_cuda, cin, cout, W):
 @pytest.mark.parametrize("cin,shape", [(32, (2, 6, 12, 96)), (96, (2, 6, 12, 96)),
                                        (64, (1, 2, 4, 96))])
 def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
-    """Even-depth shapes, which take the LDS-halo kernels (cout 32: the lean one,
-    two tiles per block): interior and border tiles, both batch items, one to
-    three 32-channel chunks, and an odd tile count (3 tiles: the last block has
-    one)."""
+    """Even-depth shapes, which take the LDS-halo kernel (two tiles per block;
+    cout 128 as two 64-channel passes): interior and border tiles, both batch
+    items, one to three 32-channel chunks, and an odd tile count (3 tiles: the
+    last block has one)."""
     torch = torch_cuda
     rng = np.random.default_rng(cin + shape[0])
-    for cout in (32, 64):
+    for cout in (32, 64, 128):
         x = R.bf16_round(rng.standard_normal(shape + (cin,)))
         w = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
         b = rng.standard_normal(cout).astype(np.float32)
