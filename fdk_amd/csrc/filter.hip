@@ -467,16 +467,28 @@ __device__ __forceinline__ StoreCtx store_ctx(const FilterArgs& a, int view) {
   return c;
 }
 
-// Stores output sample u of rows ra (fa) and ra+1 (fb) in the mode's form and
-// layout, and tracks their min/max.
-__device__ __forceinline__ void store_pair(const FilterArgs& a, StoreCtx& c, int view, int u, int ra,
-                                           bool has_b, float fa, float fb) {
-  if (a.qmode == kStoreNone) {
-  } else if (a.qmode == kStoreF16) {
-    store16<kStoreF16>(a, view, u, ra, has_b, fa, fb, c.qlo, c.qinv);
-  } else if (a.qmode == kStoreBF16) {
-    store16<kStoreBF16>(a, view, u, ra, has_b, fa, fb, c.qlo, c.qinv);
-  } else {
+// Per-view min/max: a wave-level reduction, then one atomic per wave.
+__device__ __forceinline__ void store_minmax(const FilterArgs& a, StoreCtx& c, int view) {
+  if (!a.minmax) return;
+  float lo = c.lo, hi = c.hi;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, o));
+    hi = fmaxf(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&a.minmax[2 * view], f2ord(lo));
+    atomicMax(&a.minmax[2 * view + 1], f2ord(hi));
+  }
+}
+
+// Stores output sample u of rows ra (fa) and ra+1 (fb) in store mode Q's form
+// and layout, and with MM tracks their min/max.
+template <int Q, bool MM>
+__device__ __forceinline__ void store_pair_q(const FilterArgs& a, StoreCtx& c, int view, int u, int ra,
+                                             bool has_b, float fa, float fb) {
+  if constexpr (Q == kStoreF16 || Q == kStoreBF16) {
+    store16<Q>(a, view, u, ra, has_b, fa, fb, c.qlo, c.qinv);
+  } else if constexpr (Q == kStoreF32) {
     float* d = a.dst + view * a.dst_view;
     if (a.transpose) {
       float* o = d + (long long)u * a.nrows + ra;
@@ -491,23 +503,40 @@ __device__ __forceinline__ void store_pair(const FilterArgs& a, StoreCtx& c, int
       if (has_b) d[(long long)(ra + 1) * a.dst_pitch + u] = fb;
     }
   }
-  c.lo = fminf(c.lo, has_b ? fminf(fa, fb) : fa);
-  c.hi = fmaxf(c.hi, has_b ? fmaxf(fa, fb) : fa);
+  if constexpr (MM) {
+    c.lo = fminf(c.lo, has_b ? fminf(fa, fb) : fa);
+    c.hi = fmaxf(c.hi, has_b ? fmaxf(fa, fb) : fa);
+  }
 }
 
-// Per-view min/max: a wave-level reduction, then one atomic per wave.
-__device__ __forceinline__ void store_minmax(const FilterArgs& a, StoreCtx& c, int view) {
-  if (!a.minmax) return;
-  float lo = c.lo, hi = c.hi;
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = fminf(lo, __shfl_xor(lo, o));
-    hi = fmaxf(hi, __shfl_xor(hi, o));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&a.minmax[2 * view], f2ord(lo));
-    atomicMax(&a.minmax[2 * view + 1], f2ord(hi));
-  }
+// The same with the mode and min/max tracking chosen at run time.
+__device__ __forceinline__ void store_pair(const FilterArgs& a, StoreCtx& c, int view, int u, int ra,
+                                           bool has_b, float fa, float fb) {
+  if (a.qmode == kStoreNone)
+    store_pair_q<kStoreNone, true>(a, c, view, u, ra, has_b, fa, fb);
+  else if (a.qmode == kStoreF16)
+    store_pair_q<kStoreF16, true>(a, c, view, u, ra, has_b, fa, fb);
+  else if (a.qmode == kStoreBF16)
+    store_pair_q<kStoreBF16, true>(a, c, view, u, ra, has_b, fa, fb);
+  else
+    store_pair_q<kStoreF32, true>(a, c, view, u, ra, has_b, fa, fb);
 }
+
+// A thread's E output points x[i] (point tid + 256 i) cropped, scaled and
+// stored: one loop per store mode, so no per-point mode branches remain.
+template <int Q, bool MM, int E>
+__device__ __forceinline__ void store_points(const FilterArgs& a, int view, int ra, bool has_b,
+                                             int off, const float2 (&x)[E]) {
+  const float inv = 1.0f / (float)a.N;
+  StoreCtx sc = store_ctx(a, view);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int u = (int)threadIdx.x + 256 * i - off;
+    if (u >= 0 && u < a.nu) store_pair_q<Q, MM>(a, sc, view, u, ra, has_b, x[i].x * inv, x[i].y * inv);
+  }
+  if constexpr (MM) store_minmax(a, sc, view);
+}
+
 
 // M = log2 N selects the Stockham plan (9..13); M = 0 runs the in-place
 // bit-reversed radix-2^2 FFT for the other lengths.
@@ -600,14 +629,16 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs a) {
     }
     filter_sk<M>(buf, a.tw, a.H, x);
     // x[i] is now output point tid + 256 i: crop, scale and store it
-    const float inv = 1.0f / (float)a.N;
-    StoreCtx sc = store_ctx(a, view);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const int u = (int)threadIdx.x + 256 * i - off;
-      if (u >= 0 && u < nu) store_pair(a, sc, view, u, ra, has_b, x[i].x * inv, x[i].y * inv);
-    }
-    store_minmax(a, sc, view);
+    if (a.qmode == kStoreNone)
+      store_points<kStoreNone, true>(a, view, ra, has_b, off, x);
+    else if (a.qmode == kStoreF16)
+      store_points<kStoreF16, false>(a, view, ra, has_b, off, x);
+    else if (a.qmode == kStoreBF16)
+      store_points<kStoreBF16, false>(a, view, ra, has_b, off, x);
+    else if (a.minmax)
+      store_points<kStoreF32, true>(a, view, ra, has_b, off, x);
+    else
+      store_points<kStoreF32, false>(a, view, ra, has_b, off, x);
     return;
   } else {
     for (int n = threadIdx.x; n < a.N; n += blockDim.x) {
