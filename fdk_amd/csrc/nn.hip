@@ -759,6 +759,108 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(Wgrad a) {
   }
 }
 
+// Weight gradient on the forward kernel's tiles (2 planes x 4 rows x 32 voxels):
+// a block owns one 32-channel chunk of cout and of cin and walks tiles; per
+// tile it stages the dY chunk (256 voxels x 32 channels) and the input halo
+// chunk (4 planes x 6 rows x 34 voxels x 32 channels) in LDS in their natural
+// channels-last order -- plain 16-byte copies, no transposition -- and then
+// accumulates all 27 taps: dW[tap][co][ci] += sum over the tile's voxels v of
+// dY[v][co] X[v + tap][ci], with K = 32 voxels of one tile row per MFMA.
+//
+// The operands come out of LDS through gfx950's transposing read
+// ds_read_b64_tr_b16: within each 16-lane group, source lane s loads 4
+// consecutive 16-bit values R[s][0..3] and lane l receives R[4 j + l / 4][l % 4],
+// j = 0..3 (lane layout checked on the device).  Pointing source lane s at
+// voxel x0 + s / 4, channels c0 + 4 (s % 4) .. + 3 of a [voxel][channel] tile
+// hands lane l the values of channel c0 + l for voxels x0 .. x0 + 3: the
+// 8-voxel K run of an MFMA operand is two such reads.  A tap's x shift is only
+// a different voxel row in those addresses.
+//
+// Wave w computes co tile (w & 1) x ci tile (w >> 1) of the chunk pair for all
+// 27 taps (27 accumulators); per tile row it reads its dY fragment once and
+// each tap's input fragment, and each block adds its partial sums to dW with
+// one FP32 atomic per element at the end.  Needs cin % 32 == cout % 32 == 0,
+// W % 32, H % 4 and D % 2 == 0.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+constexpr int kWgDy = 256 * 4;  // 16-byte chunks of the dY tile (256 voxels x 32 channels)
+
+__device__ __forceinline__ bf16x8 tr_frag(const unsigned char* lds, unsigned off_lo, unsigned off_hi) {
+  typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+  const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(lds + off_lo));
+  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(lds + off_hi));
+  typedef short s16x8 __attribute__((ext_vector_type(8)));
+  const s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__global__ __launch_bounds__(256) void conv3d_wgrad_tr_kernel(Wgrad g, int tiles_x, int tiles_y,
+                                                              int ntiles) {
+  __shared__ uint4 halo[kHaloChunks];
+  __shared__ uint4 dyt[kWgDy];
+  const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+  const int kq = lane / 16, sl = lane % 16;  // 16-lane group and the lane within it
+  const int ncoc = g.cout / 32;
+  const int co0 = (blockIdx.y % ncoc) * 32, ci0 = (blockIdx.y / ncoc) * 32;
+  const int ct = wave & 1, it = wave >> 1;
+  // the forward kernel's tile and halo helpers, on the input tensor
+  Conv a{g.x, nullptr, nullptr, nullptr, nullptr, g.n, g.d, g.h, g.w, g.cin, 32, 0, 0};
+  const unsigned char* hb = reinterpret_cast<const unsigned char*>(halo);
+  const unsigned char* db = reinterpret_cast<const unsigned char*>(dyt);
+  // per-lane byte offsets of the transposing reads: voxel x0 + sl / 4 (+ 4 for
+  // the second half), channel 4 (sl % 4) of the wave's 16-channel tile
+  const unsigned a_lane = ((8 * kq + sl / 4) * 32 + ct * 16 + 4 * (sl % 4)) * 2;
+  const unsigned b_lane = ((8 * kq + sl / 4) * 32 + it * 16 + 4 * (sl % 4)) * 2;
+  f32x4 acc[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const Fwd4Tile f = fwd4_tile(a, tile, tiles_x, tiles_y);
+    uint4 pre[kHaloPer];
+    fwd4_fetch(pre, a, f, ci0);
+    // dY chunk: slot s = t + 256 j is tile row s / 128 (plane (s / 128) / 4, row
+    // (s / 128) % 4), voxel (s % 128) / 4, 8-channel group s % 4; a tile row
+    // is 32 voxels x 64 bytes, so the slots of one row are contiguous
+    const u16* dyo = g.dy + (((long long)f.zb * g.h + f.y0) * g.w + f.x0) * g.cout + co0;
+    const long long plane_e = (long long)g.h * g.w * g.cout, row_e = (long long)g.w * g.cout;
+    const int pr0 = t >> 7, off0 = ((t & 127) >> 2) * g.cout + 8 * (t & 3);
+    const uint4 d0 = *reinterpret_cast<const uint4*>(dyo + (pr0 >> 2) * plane_e + (pr0 & 3) * row_e + off0);
+    const uint4 d1 = *reinterpret_cast<const uint4*>(dyo + ((pr0 + 2) >> 2) * plane_e + ((pr0 + 2) & 3) * row_e + off0);
+    const uint4 d2 = *reinterpret_cast<const uint4*>(dyo + ((pr0 + 4) >> 2) * plane_e + ((pr0 + 4) & 3) * row_e + off0);
+    const uint4 d3 = *reinterpret_cast<const uint4*>(dyo + ((pr0 + 6) >> 2) * plane_e + ((pr0 + 6) & 3) * row_e + off0);
+    __syncthreads();  // the previous tile's reads are done
+#pragma unroll
+    for (int j = 0; j < kHaloPer; ++j)
+      if (j + 1 < kHaloPer || t + 256 * j < kHaloChunks) halo[t + 256 * j] = pre[j];
+    dyt[t] = d0;
+    dyt[t + 256] = d1;
+    dyt[t + 512] = d2;
+    dyt[t + 768] = d3;
+    __syncthreads();
+#pragma unroll 1
+    for (int kc = 0; kc < 8; ++kc) {  // tile row: plane kc / 4, row kc % 4
+      const int pz = kc >> 2, py = kc & 3;
+      const unsigned ao = a_lane + (unsigned)kc * (32 * 32 * 2);
+      const bf16x8 af = tr_frag(db, ao, ao + 4 * 32 * 2);
+#pragma unroll
+      for (int tap = 0; tap < 27; ++tap) {
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const unsigned bo =
+            b_lane + (unsigned)((((pz + dz) * kHaloRows + py + dy) * kHaloX + dx) * (32 * 2));
+        const bf16x8 bfr = tr_frag(hb, bo, bo + 4 * 32 * 2);
+        acc[tap] = mfma(af, bfr, acc[tap]);
+      }
+    }
+  }
+  // lane (sl, kq) holds co = co0 + 16 ct + 4 kq + i, ci = ci0 + 16 it + sl
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int co = co0 + ct * 16 + 4 * kq + i, ci = ci0 + it * 16 + sl;
+      atomicAdd(&g.dw[((long long)tap * g.cout + co) * g.cin + ci], acc[tap][i]);
+    }
+}
+
 // Weight gradient of the cin == 1 convolution: dW[co][tap] (tap < 27, row
 // length 32) += sum_v dY[v][co] X[v + tap].  FP32 vector math.
 __global__ __launch_bounds__(256) void conv3d_wgrad_c1_kernel(Wgrad a) {
@@ -1460,6 +1562,16 @@ extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int
     return FDK_OK;
   }
   FDK_REQUIRE(cin % 16 == 0 && wd % 32 == 0, "cin %% 16 and W %% 32 must be 0");
+  if (conv_kernel_choice() >= 4 && cin % 32 == 0 && cout % 32 == 0 && h % 4 == 0 && d % 2 == 0 &&
+      aligned16(x) && aligned16(dy) && (4LL * h * wd + 64) * cin < (1LL << 31)) {
+    const int tiles_x = wd / 32, tiles_y = h / 4, vt = tiles_x * tiles_y * (n * d / 2);
+    const int chunks = (cout / 32) * (cin / 32);
+    // about 1024 blocks: every CU busy several times over, few partial sums to add
+    const int gx = std::max(1, std::min(vt, 1024 / chunks));
+    hipLaunchKernelGGL(conv3d_wgrad_tr_kernel, dim3(gx, chunks), dim3(256), 0, st, a, tiles_x, tiles_y, vt);
+    FDK_HIP(hipGetLastError());
+    return FDK_OK;
+  }
   const int ntiles = (cout / 16) * (cin / 16);
   const int per_wave = (ntiles + 3) / 4;
   const size_t lds = (size_t)(cout + cin) * (wd + 8) * sizeof(u16);

@@ -179,6 +179,23 @@ def test_conv3d_input_and_weight_gradients(torch_cuda, cin, cout, W):
     assert rel(dw.cpu().numpy(), 2 * R.conv3d_dw(x, dy)) < 1e-5
 
 
+@pytest.mark.parametrize("cin,cout,shape", [(32, 32, (2, 4, 8, 64)), (96, 32, (1, 6, 12, 96)),
+                                            (64, 128, (2, 2, 4, 32)), (192, 64, (1, 4, 8, 64)),
+                                            (32, 64, (1, 6, 12, 96))])
+def test_conv3d_weight_gradient_tiled(torch_cuda, cin, cout, shape):
+    """The tiled weight gradient (32-channel chunk pairs, all 27 taps per block,
+    operands through the transposing LDS read): interior and border tiles, both
+    batch items, several chunks of cin and cout, accumulation into dW."""
+    torch = torch_cuda
+    rng = np.random.default_rng(cin + 3 * cout + shape[0])
+    x = R.bf16_round(rng.standard_normal(shape + (cin,)))
+    dy = R.bf16_round(rng.standard_normal(shape + (cout,)))
+    dw = torch.zeros((27, cout, cin), device="cuda")
+    U.conv3d_wgrad(bf(torch, x), bf(torch, dy), dw, shape, cin, cout)
+    U.conv3d_wgrad(bf(torch, x), bf(torch, dy), dw, shape, cin, cout)  # accumulates
+    assert rel(dw.cpu().numpy(), 2 * R.conv3d_dw(x, dy)) < 1e-5
+
+
 def test_conv3d_weight_gradient_single_input_channel(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(9)
