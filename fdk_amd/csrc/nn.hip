@@ -360,19 +360,33 @@ struct Fwd4Tile {
   bool interior;          // the whole halo lies inside the volume
 };
 
+// Tile `tile` of TP planes x TR rows x 32 voxels (x fastest, then rows, then
+// the flattened n * d planes; d % TP == 0).
+template <int TP = 2, int TR = 4>
 __device__ __forceinline__ Fwd4Tile fwd4_tile(const Conv& a, int tile, int tiles_x, int tiles_y) {
   Fwd4Tile f;
   const int tx = tile % tiles_x, r = tile / tiles_x;
   const int ty = r % tiles_y;
-  f.zb = (r / tiles_y) * 2;
+  f.zb = (r / tiles_y) * TP;
   f.x0 = tx * 32;
-  f.y0 = ty * 4;
+  f.y0 = ty * TR;
   f.n = f.zb / a.d;
   f.z0 = f.zb - f.n * a.d;
-  f.interior = f.z0 >= 1 && f.z0 + 3 <= a.d && f.y0 >= 1 && f.y0 + 5 <= a.h && f.x0 >= 1 &&
-               f.x0 + 33 <= a.w;
+  f.interior = f.z0 >= 1 && f.z0 + TP + 1 <= a.d && f.y0 >= 1 && f.y0 + TR + 1 <= a.h &&
+               f.x0 >= 1 && f.x0 + 33 <= a.w;
   return f;
 }
+
+// Halo of a TP x TR x 32 tile: (TP + 2) planes x (TR + 2) rows x 34 voxels,
+// 16-byte chunks of 8 channels, handed out to 256 threads PER at a time.
+template <int TP, int TR>
+struct HaloShape {
+  static constexpr int rows = TR + 2;                         // halo rows per plane
+  static constexpr int chunks = (TP + 2) * rows * 4 * kHaloX;  // 16-byte chunks
+  static constexpr int per = (chunks + 255) / 256;
+};
+template <int TP, int TR>
+using HaloRegs = uint4[HaloShape<TP, TR>::per];  // one thread's share of the halo
 
 // Loads the halo of (tile, 32-channel chunk c0) into registers: slot s = t + 256 j
 // is halo row s / 136 (plane-major), x s % 136 / 4 and 8-channel group s % 4.
@@ -382,8 +396,11 @@ __device__ __forceinline__ Fwd4Tile fwd4_tile(const Conv& a, int tile, int tiles
 // upsampled half (uniform) reads the coarse tensor `up`: full-resolution voxel
 // z0 - 1 + hz maps to coarse (z0 - 1 + hz) >> 1 = z0 / 2 + ((hz - 1) >> 1) (z0,
 // y0 and x0 are even), the same offset formula with every step halved.
-__device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a, const Fwd4Tile& f,
+template <int TP = 2, int TR = 4>
+__device__ __forceinline__ void fwd4_fetch(HaloRegs<TP, TR>& pre, const Conv& a, const Fwd4Tile& f,
                                            int c0) {
+  constexpr int kRows = HaloShape<TP, TR>::rows, kPer = HaloShape<TP, TR>::per,
+                kChunks = HaloShape<TP, TR>::chunks;
   const int t = threadIdx.x;
   const bool coarse = a.up != nullptr && c0 < a.cup;
   const int sh = coarse ? 1 : 0;
@@ -395,9 +412,9 @@ __device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a
   const int plane = hh * ww * cc, rowe = ww * cc;
   int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
 #pragma unroll
-  for (int j = 0; j < kHaloPer; ++j) {
-    const int hz = row / kHaloRows, hy = row - hz * kHaloRows, hx = w >> 2;
-    bool ok = j + 1 < kHaloPer || t + 256 * j < kHaloChunks;
+  for (int j = 0; j < kPer; ++j) {
+    const int hz = row / kRows, hy = row - hz * kRows, hx = w >> 2;
+    bool ok = j + 1 < kPer || t + 256 * j < kChunks;
     if (!f.interior) {
       const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
       ok = ok && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
@@ -413,12 +430,14 @@ __device__ __forceinline__ void fwd4_fetch(uint4 (&pre)[kHaloPer], const Conv& a
   }
 }
 
-__device__ __forceinline__ void fwd4_commit(uint4* halo, const uint4 (&pre)[kHaloPer]) {
+template <int TP = 2, int TR = 4>
+__device__ __forceinline__ void fwd4_commit(uint4* halo, const HaloRegs<TP, TR>& pre) {
+  constexpr int kPer = HaloShape<TP, TR>::per, kChunks = HaloShape<TP, TR>::chunks;
   const int t = threadIdx.x;
   int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
 #pragma unroll
-  for (int j = 0; j < kHaloPer; ++j) {
-    if (t + 256 * j < kHaloChunks) halo[(row * 4 + (w & 3)) * kHaloX + (w >> 2)] = pre[j];
+  for (int j = 0; j < kPer; ++j) {
+    if (t + 256 * j < kChunks) halo[(row * 4 + (w & 3)) * kHaloX + (w >> 2)] = pre[j];
     w += 256 - kHaloRowSlots;
     row += 1;
     if (w >= kHaloRowSlots) {
@@ -548,6 +567,205 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
           *reinterpret_cast<uint2*>(static_cast<u16*>(a.y) + o) = pack4_bf16(v[0], v[1], v[2], v[3]);
       }
     }
+  }
+}
+
+// conv3d_fwd5_kernel: the LDS-halo convolution with each A fragment (16 input
+// voxels x 32 channels of one halo row at one x shift) read once per weight
+// slice and applied to every output row that uses it.
+//  * tile: 4 planes x Q rows x 32 voxels, wave w = output plane w, all Q rows,
+//    both 16-voxel halves and NT 16-channel output tiles (run at cout 32: Q =
+//    4, NT = 2); halo (6 planes x (Q + 2) rows x 34, 78 KB at Q = 4) in dynamic
+//    LDS, the wave reading planes w..w+2;
+//  * the 27 taps run as 9 slices (dx, dz), each 3 dy taps x NT weight
+//    fragments held in registers (one slice loaded ahead); in a slice, input
+//    row yi (0..Q+1) of plane w + dz feeds output rows yi - dy, 0 <= yi - dy < Q;
+//  * per 32-channel chunk and wave: 18 (Q + 2) A fragments for 54 Q NT MFMAs
+//    (conv3d_fwd4_kernel: 108 for 216 NT), i.e. 0.25 LDS fragment reads per
+//    MFMA at cout 32 instead of 0.5.
+// Needs d % 4 == 0 and h % Q == 0 besides conv3d_fwd4_kernel's rules.
+template <int NT>
+__device__ __forceinline__ void fwd5_load_slice(bf16x8 (&b)[3][NT], __amdgpu_buffer_rsrc_t wr,
+                                                unsigned lane_off, unsigned o, unsigned nt_bytes,
+                                                unsigned dy_bytes) {
+#pragma unroll
+  for (int dy = 0; dy < 3; ++dy) {
+    unsigned p = o;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      b[dy][nt] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(wr, lane_off, p, 0));
+      p += nt_bytes;
+      asm volatile("" : "+s"(p));
+    }
+    o += dy_bytes;
+    asm volatile("" : "+s"(o));
+  }
+}
+
+// fwd4_fetch through a buffer resource based at the tile's lowest halo element
+// (hz, hy, hx = 0): one 32-bit offset per slot instead of a 64-bit address, and
+// slots outside the volume take an offset past the resource's end, which the
+// buffer range check turns into zeros, instead of a branch.  The resource
+// spans TP + 2 planes and 2 rows of the source (< 2^32 - 16 bytes: checked by
+// the launcher).
+template <int TP, int TR>
+__device__ __forceinline__ void fwd5_fetch(HaloRegs<TP, TR>& pre, const Conv& a, const Fwd4Tile& f,
+                                           int c0) {
+  constexpr int kRows = HaloShape<TP, TR>::rows, kPer = HaloShape<TP, TR>::per,
+                kChunks = HaloShape<TP, TR>::chunks;
+  const int t = threadIdx.x;
+  const bool coarse = a.up != nullptr && c0 < a.cup;
+  const int sh = coarse ? 1 : 0;
+  const int cc = coarse ? a.cup : a.cin - a.cup;
+  const int hh = a.h >> sh, ww = a.w >> sh;
+  const int plane = hh * ww * cc, rowe = ww * cc;
+  const long long lo = (((long long)f.n * (a.d >> sh) + (f.z0 >> sh)) * hh + (f.y0 >> sh)) * ww +
+                       (f.x0 >> sh) - ((long long)hh * ww + ww + 1);  // voxel (hz, hy, hx) = 0
+  const u16* base = (coarse ? a.up : a.x) + lo * cc + (coarse ? c0 : c0 - a.cup);
+  // in-volume slots lie within TP + 2 planes + 2 rows of the base
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)base, 0, (int)(((unsigned)(TP + 2) * (unsigned)plane + 2u * (unsigned)rowe) * 2u),
+      fdk::kRawRsrcWord3);
+  int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
+  // opaque: the slot coordinates are recomputed per call, not held in registers
+  // across the tap loop
+  asm volatile("" : "+v"(row), "+v"(w));
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int hz = row / kRows, hy = row - hz * kRows, hx = w >> 2;
+    bool ok = j + 1 < kPer || t + 256 * j < kChunks;
+    if (!f.interior) {
+      const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
+      ok = ok && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    }
+    // fwd4_fetch's ((h - 1) >> sh) + 1 == (h + sh) >> sh for h >= 0, sh in {0, 1}
+    const unsigned off = ((unsigned)((hz + sh) >> sh) * (unsigned)plane +
+                          (unsigned)(((hy + sh) >> sh) * rowe + ((hx + sh) >> sh) * cc + 8 * (w & 3))) *
+                         2u;
+    pre[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? off : 0xFFFFFFF0u, 0, 0));
+    w += 256 - kHaloRowSlots;
+    row += 1;
+    if (w >= kHaloRowSlots) {
+      w -= kHaloRowSlots;
+      row += 1;
+    }
+  }
+}
+
+// A fragment of step i (slice i / K: dx = slice / 3, dz = slice % 3; input row
+// (i % K) / 2, half i % 2) from the wave's halo window.
+template <int Q>
+__device__ __forceinline__ bf16x8 fwd5_read_a(const uint4* hb, int i) {
+  constexpr int K = 2 * (Q + 2);
+  const int sl = i / K, k = i % K;
+  const int dx = sl / 3, dz = sl % 3, yi = k / 2, mt = k % 2;
+  return __builtin_bit_cast(bf16x8, hb[(dz * (Q + 2) + yi) * (4 * kHaloX) + 16 * mt + dx]);
+}
+
+template <int NT, int Q, bool PF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_fwd5_kernel(
+    Conv a, int tiles_x, int tiles_y, int ntiles, int tpb) {
+  using HS = HaloShape<4, Q>;
+  extern __shared__ uint4 halo5[];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int m = lane % 16, kq = lane / 16;
+  const uint4* hb = halo5 + (wave * HS::rows * 4 + kq) * kHaloX + m;
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), fdk::kRawRsrcWord3);
+  const unsigned lane_off = (unsigned)(kq * 8 + (a.co0 + m) * a.cin) * 2u;
+  const unsigned tap_bytes = (unsigned)(a.cout * a.cin) * 2u, nt_bytes = (unsigned)(16 * a.cin) * 2u;
+  const int nchunks = a.cin / 32;
+  const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
+  uint4 pre[HS::per];
+  if (PF && first < last) fwd5_fetch<4, Q>(pre, a, fwd4_tile<4, Q>(a, first, tiles_x, tiles_y), 0);
+  for (int tile = first; tile < last; ++tile) {
+    const Fwd4Tile f = fwd4_tile<4, Q>(a, tile, tiles_x, tiles_y);
+    f32x4 acc[Q][2][NT];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[q][mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ch = 0; ch < nchunks; ++ch) {
+      if (!PF) fwd5_fetch<4, Q>(pre, a, f, ch * 32);
+      __syncthreads();  // every wave is done reading the previous halo
+      fwd4_commit<4, Q>(halo5, pre);
+      __syncthreads();
+      const unsigned cb = (unsigned)ch * 64u;  // 32 channels x 2 bytes
+      bf16x8 b[2][3][NT];
+      fwd5_load_slice<NT>(b[0], wr, lane_off, cb, nt_bytes, 3u * tap_bytes);
+      if (PF) {
+        int nt_ = tile, nc = ch + 1;
+        if (nc == nchunks) {
+          nc = 0;
+          ++nt_;
+        }
+        if (nt_ < last)
+          fwd5_fetch<4, Q>(pre, a, nt_ == tile ? f : fwd4_tile<4, Q>(a, nt_, tiles_x, tiles_y), nc * 32);
+      }
+      // steps i = (slice, input row, half): A fragments kAhead steps ahead of
+      // their MFMAs, the next slice's weights at the first step of a slice
+      constexpr int K = 2 * (Q + 2), kSteps = 9 * K, kAhead = 2;
+      bf16x8 ar[kAhead + 1];
+#pragma unroll
+      for (int i = 0; i < kAhead; ++i) ar[i] = fwd5_read_a<Q>(hb, i);
+#pragma unroll
+      for (int sl = 0; sl < 9; ++sl) {
+        if (sl + 1 < 9) {
+          const int dx1 = (sl + 1) / 3, dz1 = (sl + 1) % 3;
+          unsigned o = (unsigned)(dz1 * 9 + dx1) * tap_bytes;
+          asm volatile("" : "+s"(o));
+          fwd5_load_slice<NT>(b[(sl + 1) & 1], wr, lane_off, cb + o, nt_bytes, 3u * tap_bytes);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int i = sl * K + k, yi = k / 2, mt = k % 2;
+          if (i + kAhead < kSteps) ar[(i + kAhead) % (kAhead + 1)] = fwd5_read_a<Q>(hb, i + kAhead);
+          __builtin_amdgcn_sched_barrier(0);
+          const bf16x8 af = ar[i % (kAhead + 1)];
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy) {
+            const int q = yi - dy;
+            if (q < 0 || q >= Q) continue;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[q][mt][nt] = mfma(b[sl & 1][dy][nt], af, acc[q][mt][nt]);
+          }
+        }
+      }
+    }
+    // lane (m, kq) holds channels nt * 16 + 4 kq + i of voxel m of half mt of row q
+    const int zo = f.zb + wave;
+    float4 bias[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + a.co0 + nt * 16 + 4 * kq)
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        const long long vox = ((long long)zo * a.h + f.y0 + q) * a.w + f.x0 + 16 * mt + m;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const long long o = vox * a.cout + a.co0 + nt * 16 + 4 * kq;
+          float v[4] = {acc[q][mt][nt][0] + bias[nt].x, acc[q][mt][nt][1] + bias[nt].y,
+                        acc[q][mt][nt][2] + bias[nt].z, acc[q][mt][nt][3] + bias[nt].w};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = activate(v[i], a.relu);
+          if (a.resid) {
+            const float4 r = *reinterpret_cast<const float4*>(a.resid + o);
+            v[0] += r.x;
+            v[1] += r.y;
+            v[2] += r.z;
+            v[3] += r.w;
+          }
+          if (a.out_f32)
+            *reinterpret_cast<float4*>(static_cast<float*>(a.y) + o) = make_float4(v[0], v[1], v[2], v[3]);
+          else
+            *reinterpret_cast<uint2*>(static_cast<u16*>(a.y) + o) = pack4_bf16(v[0], v[1], v[2], v[3]);
+        }
+      }
   }
 }
 
@@ -1434,16 +1652,18 @@ extern "C" int fdk_nn_mfma_probe(const float* A, const float* B, float* D, void*
   return FDK_OK;
 }
 
-// Forward convolution kernel: the lean LDS-halo kernel (cout 32 / 64) or the
-// LDS-halo kernel where their shape rules hold (default), else the pipelined
-// 2-row tile, else the 1-row kernel; FDK_NN_CONV=3, 2 or 1 caps the choice.
+// Forward convolution kernel: the row-reuse LDS-halo kernel (5, default), the
+// lean LDS-halo kernel (4; both cout 32 / 64 passes) or the LDS-halo kernel
+// where their shape rules hold, else the pipelined 2-row tile, else the 1-row
+// kernel; FDK_NN_CONV=4, 3, 2 or 1 caps the choice.
 static int conv_kernel_choice() {
   static const int k = [] {
     const char* e = std::getenv("FDK_NN_CONV");
     if (e && e[0] == '1') return 1;
     if (e && e[0] == '2') return 2;
     if (e && e[0] == '3') return 3;
-    return 4;
+    if (e && e[0] == '4') return 4;
+    return 5;
   }();
   return k;
 }
@@ -1461,7 +1681,31 @@ static bool lean_ok(const Conv& a) {
          27LL * a.cout * a.cin * 2 < (1LL << 31);
 }
 
+template <int NT, int Q, bool PF>
+static int launch_fwd5_pass(const Conv& a, hipStream_t st) {
+  const void* fn = reinterpret_cast<const void*>(&conv3d_fwd5_kernel<NT, Q, PF>);
+  const int lds = HaloShape<4, Q>::chunks * (int)sizeof(uint4);
+  FDK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  const int tiles_x = a.w / 32, tiles_y = a.h / Q, ntiles = tiles_x * tiles_y * (a.n * a.d / 4);
+  const int tpb = 2;
+  hipLaunchKernelGGL((conv3d_fwd5_kernel<NT, Q, PF>), dim3((ntiles + tpb - 1) / tpb), dim3(256), lds, st,
+                     a, tiles_x, tiles_y, ntiles, tpb);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+// conv3d_fwd5_kernel at cout 32 where its tile fits (d % 4 == 0, h % 4 == 0)
+// and its halo resource (6 planes + 2 rows) stays below 2^32 - 16 bytes.  (At cout 64 the row-reuse form,
+// 4 planes x 2 rows with four 16-channel tiles per wave, ran no faster than
+// conv3d_fwd4_kernel: its weight fragments, not its LDS reads, dominate.)
+static bool fwd5_ok(const Conv& a) {
+  const int cx = a.cin - a.cup;
+  return conv_kernel_choice() >= 5 && a.cout == 32 && a.d % 4 == 0 && a.h % 4 == 0 &&
+         (6LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8;
+}
+
 static int launch_fwd4(const Conv& a, hipStream_t st) {
+  if (fwd5_ok(a)) return launch_fwd5_pass<2, 4, true>(a, st);
   const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / 2);
   const int tpb = 2;
   const dim3 g4((ntiles + tpb - 1) / tpb);

@@ -63,12 +63,14 @@ def test_conv3d_forward(torch_cuda, cin, cout, W):
 
 
 @pytest.mark.parametrize("cin,shape", [(32, (2, 6, 12, 96)), (96, (2, 6, 12, 96)),
-                                       (64, (1, 2, 4, 96))])
+                                       (64, (1, 2, 4, 96)), (32, (2, 8, 12, 96)),
+                                       (96, (1, 4, 4, 32)), (64, (1, 12, 8, 64))])
 def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
-    """Even-depth shapes, which take the LDS-halo kernel (two tiles per block;
+    """Even-depth shapes, which take the LDS-halo kernels (two tiles per block;
     cout 128 as two 64-channel passes): interior and border tiles, both batch
     items, one to three 32-channel chunks, and an odd tile count (3 tiles: the
-    last block has one)."""
+    last block has one).  Depth 6 and 2 run the 2-plane tiles; at cout 32,
+    depth 8, 4 and 12 run the 4-plane x 4-row row-reuse tiles."""
     torch = torch_cuda
     rng = np.random.default_rng(cin + shape[0])
     for cout in (32, 64, 128):
