@@ -447,6 +447,56 @@ __device__ __forceinline__ void fwd4_commit(uint4* halo, const HaloRegs<TP, TR>&
   }
 }
 
+// fwd4_fetch through a buffer resource based at the tile's lowest halo element
+// (hz, hy, hx = 0): one 32-bit offset per slot instead of a 64-bit address, and
+// slots outside the volume take an offset past the resource's end, which the
+// buffer range check turns into zeros, instead of a branch.  The resource
+// spans TP + 2 planes and 2 rows of the source (< 2^32 - 16 bytes: checked by
+// the launcher).
+template <int TP, int TR>
+__device__ __forceinline__ void fwd5_fetch(HaloRegs<TP, TR>& pre, const Conv& a, const Fwd4Tile& f,
+                                           int c0) {
+  constexpr int kRows = HaloShape<TP, TR>::rows, kPer = HaloShape<TP, TR>::per,
+                kChunks = HaloShape<TP, TR>::chunks;
+  const int t = threadIdx.x;
+  const bool coarse = a.up != nullptr && c0 < a.cup;
+  const int sh = coarse ? 1 : 0;
+  const int cc = coarse ? a.cup : a.cin - a.cup;
+  const int hh = a.h >> sh, ww = a.w >> sh;
+  const int plane = hh * ww * cc, rowe = ww * cc;
+  const long long lo = (((long long)f.n * (a.d >> sh) + (f.z0 >> sh)) * hh + (f.y0 >> sh)) * ww +
+                       (f.x0 >> sh) - ((long long)hh * ww + ww + 1);  // voxel (hz, hy, hx) = 0
+  const u16* base = (coarse ? a.up : a.x) + lo * cc + (coarse ? c0 : c0 - a.cup);
+  // in-volume slots lie within TP + 2 planes + 2 rows of the base
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)base, 0, (int)(((unsigned)(TP + 2) * (unsigned)plane + 2u * (unsigned)rowe) * 2u),
+      fdk::kRawRsrcWord3);
+  int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
+  // opaque: the slot coordinates are recomputed per call, not held in registers
+  // across the tap loop
+  asm volatile("" : "+v"(row), "+v"(w));
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int hz = row / kRows, hy = row - hz * kRows, hx = w >> 2;
+    bool ok = j + 1 < kPer || t + 256 * j < kChunks;
+    if (!f.interior) {
+      const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
+      ok = ok && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    }
+    // fwd4_fetch's ((h - 1) >> sh) + 1 == (h + sh) >> sh for h >= 0, sh in {0, 1}
+    const unsigned off = ((unsigned)((hz + sh) >> sh) * (unsigned)plane +
+                          (unsigned)(((hy + sh) >> sh) * rowe + ((hx + sh) >> sh) * cc + 8 * (w & 3))) *
+                         2u;
+    pre[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? off : 0xFFFFFFF0u, 0, 0));
+    w += 256 - kHaloRowSlots;
+    row += 1;
+    if (w >= kHaloRowSlots) {
+      w -= kHaloRowSlots;
+      row += 1;
+    }
+  }
+}
+
 // Weight fragments of one tap at the running offset `soff` (advanced by one tap;
 // the empty asm keeps it a running SGPR instead of 27 hoisted products).
 template <int NT>
@@ -491,7 +541,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const int nchunks = a.cin / 32;
   const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
   uint4 pre[kHaloPer];
-  if (PF && first < last) fwd4_fetch(pre, a, fwd4_tile(a, first, tiles_x, tiles_y), 0);
+  if (PF && first < last) fwd5_fetch<2, 4>(pre, a, fwd4_tile(a, first, tiles_x, tiles_y), 0);
   for (int tile = first; tile < last; ++tile) {
     const Fwd4Tile f = fwd4_tile(a, tile, tiles_x, tiles_y);
     f32x4 acc[4][NT];
@@ -500,7 +550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ch = 0; ch < nchunks; ++ch) {
-      if (!PF) fwd4_fetch(pre, a, f, ch * 32);  // staged synchronously: no registers held
+      if (!PF) fwd5_fetch<2, 4>(pre, a, f, ch * 32);  // staged synchronously: no registers held
       __syncthreads();  // every wave is done reading the previous halo
       fwd4_commit(halo, pre);
       __syncthreads();
@@ -515,7 +565,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
           nc = 0;
           ++nt_;
         }
-        if (nt_ < last) fwd4_fetch(pre, a, nt_ == tile ? f : fwd4_tile(a, nt_, tiles_x, tiles_y), nc * 32);
+        if (nt_ < last) fwd5_fetch<2, 4>(pre, a, nt_ == tile ? f : fwd4_tile(a, nt_, tiles_x, tiles_y), nc * 32);
       }
       // NT == 2: A fragments are read one tap ahead of their MFMAs (two register
       // sets); NT == 4 has no registers to spare and reads them in the tap.
@@ -599,56 +649,6 @@ __device__ __forceinline__ void fwd5_load_slice(bf16x8 (&b)[3][NT], __amdgpu_buf
     }
     o += dy_bytes;
     asm volatile("" : "+s"(o));
-  }
-}
-
-// fwd4_fetch through a buffer resource based at the tile's lowest halo element
-// (hz, hy, hx = 0): one 32-bit offset per slot instead of a 64-bit address, and
-// slots outside the volume take an offset past the resource's end, which the
-// buffer range check turns into zeros, instead of a branch.  The resource
-// spans TP + 2 planes and 2 rows of the source (< 2^32 - 16 bytes: checked by
-// the launcher).
-template <int TP, int TR>
-__device__ __forceinline__ void fwd5_fetch(HaloRegs<TP, TR>& pre, const Conv& a, const Fwd4Tile& f,
-                                           int c0) {
-  constexpr int kRows = HaloShape<TP, TR>::rows, kPer = HaloShape<TP, TR>::per,
-                kChunks = HaloShape<TP, TR>::chunks;
-  const int t = threadIdx.x;
-  const bool coarse = a.up != nullptr && c0 < a.cup;
-  const int sh = coarse ? 1 : 0;
-  const int cc = coarse ? a.cup : a.cin - a.cup;
-  const int hh = a.h >> sh, ww = a.w >> sh;
-  const int plane = hh * ww * cc, rowe = ww * cc;
-  const long long lo = (((long long)f.n * (a.d >> sh) + (f.z0 >> sh)) * hh + (f.y0 >> sh)) * ww +
-                       (f.x0 >> sh) - ((long long)hh * ww + ww + 1);  // voxel (hz, hy, hx) = 0
-  const u16* base = (coarse ? a.up : a.x) + lo * cc + (coarse ? c0 : c0 - a.cup);
-  // in-volume slots lie within TP + 2 planes + 2 rows of the base
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)base, 0, (int)(((unsigned)(TP + 2) * (unsigned)plane + 2u * (unsigned)rowe) * 2u),
-      fdk::kRawRsrcWord3);
-  int row = t / kHaloRowSlots, w = t - row * kHaloRowSlots;
-  // opaque: the slot coordinates are recomputed per call, not held in registers
-  // across the tap loop
-  asm volatile("" : "+v"(row), "+v"(w));
-#pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int hz = row / kRows, hy = row - hz * kRows, hx = w >> 2;
-    bool ok = j + 1 < kPer || t + 256 * j < kChunks;
-    if (!f.interior) {
-      const int zz = f.z0 - 1 + hz, yy = f.y0 - 1 + hy, xx = f.x0 - 1 + hx;
-      ok = ok && zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
-    }
-    // fwd4_fetch's ((h - 1) >> sh) + 1 == (h + sh) >> sh for h >= 0, sh in {0, 1}
-    const unsigned off = ((unsigned)((hz + sh) >> sh) * (unsigned)plane +
-                          (unsigned)(((hy + sh) >> sh) * rowe + ((hx + sh) >> sh) * cc + 8 * (w & 3))) *
-                         2u;
-    pre[j] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? off : 0xFFFFFFF0u, 0, 0));
-    w += 256 - kHaloRowSlots;
-    row += 1;
-    if (w >= kHaloRowSlots) {
-      w -= kHaloRowSlots;
-      row += 1;
-    }
   }
 }
 
@@ -1677,7 +1677,7 @@ static bool lean_ok(const Conv& a) {
   return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && (a.cout == 32 || a.cout % 64 == 0) &&
          aligned16(a.x) && aligned16(a.wt) && aligned16(a.y) && (!a.bias || aligned16(a.bias)) &&
          (!a.resid || aligned16(a.resid)) && (!a.up || aligned16(a.up)) &&
-         ((4LL * a.h * a.w + 64) * std::max(cx, a.cup) < (1LL << 31)) &&
+         ((4LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8) &&
          27LL * a.cout * a.cin * 2 < (1LL << 31);
 }
 
