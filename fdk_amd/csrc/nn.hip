@@ -623,17 +623,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 // conv3d_fwd5_kernel: the LDS-halo convolution with each A fragment (16 input
 // voxels x 32 channels of one halo row at one x shift) read once per weight
 // slice and applied to every output row that uses it.
-//  * tile: 4 planes x Q rows x 32 voxels, wave w = output plane w, all Q rows,
-//    both 16-voxel halves and NT 16-channel output tiles (run at cout 32: Q =
-//    4, NT = 2); halo (6 planes x (Q + 2) rows x 34, 78 KB at Q = 4) in dynamic
-//    LDS, the wave reading planes w..w+2;
+//  * tile: TP planes x Q rows x 32 voxels x 64 / TP output channels; wave w
+//    owns output plane w % TP, all Q rows, both 16-voxel halves and NT = 2
+//    16-channel output tiles from channel co0 + 32 (w / TP) (cout 32: TP = 4;
+//    64-channel passes: TP = 2, the two waves of a plane sharing its A
+//    fragments); halo ((TP + 2) planes x (Q + 2) rows x 34: 78 KB at TP = 4,
+//    52 KB at TP = 2) in dynamic LDS, the wave reading planes w % TP + 0..2;
 //  * the 27 taps run as 9 slices (dx, dz), each 3 dy taps x NT weight
 //    fragments held in registers (one slice loaded ahead); in a slice, input
 //    row yi (0..Q+1) of plane w + dz feeds output rows yi - dy, 0 <= yi - dy < Q;
-//  * per 32-channel chunk and wave: 18 (Q + 2) A fragments for 54 Q NT MFMAs
-//    (conv3d_fwd4_kernel: 108 for 216 NT), i.e. 0.25 LDS fragment reads per
-//    MFMA at cout 32 instead of 0.5.
-// Needs d % 4 == 0 and h % Q == 0 besides conv3d_fwd4_kernel's rules.
+//  * per 32-channel chunk and wave: 18 (Q + 2) A fragments and 27 NT weight
+//    fragments for 54 Q NT MFMAs: 0.25 LDS and 0.125 weight fragment reads per
+//    MFMA (conv3d_fwd4_kernel: 0.5 and 0.25 at cout 32, 0.25 and 0.25 at 64).
+// Needs d % TP == 0 and h % Q == 0 besides conv3d_fwd4_kernel's rules.
 template <int NT>
 __device__ __forceinline__ void fwd5_load_slice(bf16x8 (&b)[3][NT], __amdgpu_buffer_rsrc_t wr,
                                                 unsigned lane_off, unsigned o, unsigned nt_bytes,
@@ -662,24 +664,26 @@ __device__ __forceinline__ bf16x8 fwd5_read_a(const uint4* hb, int i) {
   return __builtin_bit_cast(bf16x8, hb[(dz * (Q + 2) + yi) * (4 * kHaloX) + 16 * mt + dx]);
 }
 
-template <int NT, int Q, bool PF>
+template <int NT, int Q, int TP, bool PF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void conv3d_fwd5_kernel(
     Conv a, int tiles_x, int tiles_y, int ntiles, int tpb) {
-  using HS = HaloShape<4, Q>;
+  using HS = HaloShape<TP, Q>;
   extern __shared__ uint4 halo5[];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int m = lane % 16, kq = lane / 16;
-  const uint4* hb = halo5 + (wave * HS::rows * 4 + kq) * kHaloX + m;
+  const int pz = wave % TP;                          // the wave's output plane in the tile
+  const int cw = a.co0 + (wave / TP) * (16 * NT);   // and its first output channel
+  const uint4* hb = halo5 + (pz * HS::rows * 4 + kq) * kHaloX + m;
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.wt, 0, (int)(27u * a.cout * a.cin * 2u), fdk::kRawRsrcWord3);
-  const unsigned lane_off = (unsigned)(kq * 8 + (a.co0 + m) * a.cin) * 2u;
+  const unsigned lane_off = (unsigned)(kq * 8 + (cw + m) * a.cin) * 2u;
   const unsigned tap_bytes = (unsigned)(a.cout * a.cin) * 2u, nt_bytes = (unsigned)(16 * a.cin) * 2u;
   const int nchunks = a.cin / 32;
   const int first = blockIdx.x * tpb, last = min(first + tpb, ntiles);
   uint4 pre[HS::per];
-  if (PF && first < last) fwd5_fetch<4, Q>(pre, a, fwd4_tile<4, Q>(a, first, tiles_x, tiles_y), 0);
+  if (PF && first < last) fwd5_fetch<TP, Q>(pre, a, fwd4_tile<TP, Q>(a, first, tiles_x, tiles_y), 0);
   for (int tile = first; tile < last; ++tile) {
-    const Fwd4Tile f = fwd4_tile<4, Q>(a, tile, tiles_x, tiles_y);
+    const Fwd4Tile f = fwd4_tile<TP, Q>(a, tile, tiles_x, tiles_y);
     f32x4 acc[Q][2][NT];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
@@ -688,9 +692,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[q][mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int ch = 0; ch < nchunks; ++ch) {
-      if (!PF) fwd5_fetch<4, Q>(pre, a, f, ch * 32);
+      if (!PF) fwd5_fetch<TP, Q>(pre, a, f, ch * 32);
       __syncthreads();  // every wave is done reading the previous halo
-      fwd4_commit<4, Q>(halo5, pre);
+      fwd4_commit<TP, Q>(halo5, pre);
       __syncthreads();
       const unsigned cb = (unsigned)ch * 64u;  // 32 channels x 2 bytes
       bf16x8 b[2][3][NT];
@@ -702,7 +706,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
           ++nt_;
         }
         if (nt_ < last)
-          fwd5_fetch<4, Q>(pre, a, nt_ == tile ? f : fwd4_tile<4, Q>(a, nt_, tiles_x, tiles_y), nc * 32);
+          fwd5_fetch<TP, Q>(pre, a, nt_ == tile ? f : fwd4_tile<TP, Q>(a, nt_, tiles_x, tiles_y), nc * 32);
       }
       // steps i = (slice, input row, half): A fragments kAhead steps ahead of
       // their MFMAs, the next slice's weights at the first step of a slice
@@ -735,11 +739,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
       }
     }
     // lane (m, kq) holds channels nt * 16 + 4 kq + i of voxel m of half mt of row q
-    const int zo = f.zb + wave;
+    const int zo = f.zb + pz;
     float4 bias[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
-      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + a.co0 + nt * 16 + 4 * kq)
+      bias[nt] = a.bias ? *reinterpret_cast<const float4*>(a.bias + cw + nt * 16 + 4 * kq)
                         : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
@@ -748,7 +752,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         const long long vox = ((long long)zo * a.h + f.y0 + q) * a.w + f.x0 + 16 * mt + m;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-          const long long o = vox * a.cout + a.co0 + nt * 16 + 4 * kq;
+          const long long o = vox * a.cout + cw + nt * 16 + 4 * kq;
           float v[4] = {acc[q][mt][nt][0] + bias[nt].x, acc[q][mt][nt][1] + bias[nt].y,
                         acc[q][mt][nt][2] + bias[nt].z, acc[q][mt][nt][3] + bias[nt].w};
 #pragma unroll
@@ -1681,31 +1685,38 @@ static bool lean_ok(const Conv& a) {
          27LL * a.cout * a.cin * 2 < (1LL << 31);
 }
 
-template <int NT, int Q, bool PF>
+template <int TP, bool PF>
 static int launch_fwd5_pass(const Conv& a, hipStream_t st) {
-  const void* fn = reinterpret_cast<const void*>(&conv3d_fwd5_kernel<NT, Q, PF>);
-  const int lds = HaloShape<4, Q>::chunks * (int)sizeof(uint4);
+  const void* fn = reinterpret_cast<const void*>(&conv3d_fwd5_kernel<2, 4, TP, PF>);
+  const int lds = HaloShape<TP, 4>::chunks * (int)sizeof(uint4);
   FDK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  const int tiles_x = a.w / 32, tiles_y = a.h / Q, ntiles = tiles_x * tiles_y * (a.n * a.d / 4);
+  const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / TP);
   const int tpb = 2;
-  hipLaunchKernelGGL((conv3d_fwd5_kernel<NT, Q, PF>), dim3((ntiles + tpb - 1) / tpb), dim3(256), lds, st,
-                     a, tiles_x, tiles_y, ntiles, tpb);
+  hipLaunchKernelGGL((conv3d_fwd5_kernel<2, 4, TP, PF>), dim3((ntiles + tpb - 1) / tpb), dim3(256), lds,
+                     st, a, tiles_x, tiles_y, ntiles, tpb);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
 
-// conv3d_fwd5_kernel at cout 32 where its tile fits (d % 4 == 0, h % 4 == 0)
-// and its halo resource (6 planes + 2 rows) stays below 2^32 - 16 bytes.  (At cout 64 the row-reuse form,
-// 4 planes x 2 rows with four 16-channel tiles per wave, ran no faster than
-// conv3d_fwd4_kernel: its weight fragments, not its LDS reads, dominate.)
+// conv3d_fwd5_kernel where its tile fits: cout 32 with d % 4 == 0 (4-plane
+// tiles) or cout % 64 == 0 (2-plane tiles, 64-channel passes); h % 4 == 0; its
+// halo resource ((TP + 2) planes + 2 rows) below 2^32 - 16 bytes.
 static bool fwd5_ok(const Conv& a) {
   const int cx = a.cin - a.cup;
-  return conv_kernel_choice() >= 5 && a.cout == 32 && a.d % 4 == 0 && a.h % 4 == 0 &&
-         (6LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8;
+  return conv_kernel_choice() >= 5 && ((a.cout == 32 && a.d % 4 == 0) || a.cout % 64 == 0) &&
+         a.h % 4 == 0 && (6LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8;
+}
+
+static int launch_fwd5(const Conv& a, hipStream_t st) {
+  if (a.cout == 32) return launch_fwd5_pass<4, true>(a, st);
+  Conv p = a;
+  for (p.co0 = 0; p.co0 < a.cout; p.co0 += 64)
+    if (int r = launch_fwd5_pass<2, true>(p, st)) return r;
+  return FDK_OK;
 }
 
 static int launch_fwd4(const Conv& a, hipStream_t st) {
-  if (fwd5_ok(a)) return launch_fwd5_pass<2, 4, true>(a, st);
+  if (fwd5_ok(a)) return launch_fwd5(a, st);
   const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / 2);
   const int tpb = 2;
   const dim3 g4((ntiles + tpb - 1) / tpb);
