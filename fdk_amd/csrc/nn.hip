@@ -999,12 +999,19 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(Wgrad a) {
 // a different voxel row in those addresses.
 //
 // Wave w computes co tile (w & 1) x ci tile (w >> 1) of the chunk pair for all
-// 27 taps (27 accumulators); per tile row it reads its dY fragment once and
-// each tap's input fragment, and each block adds its partial sums to dW with
-// one FP32 atomic per element at the end.  Needs cin % 32 == cout % 32 == 0,
+// 27 taps (27 accumulators); per tile it holds the dY fragments of its 8 rows
+// and reads each of the 72 input fragments (24 halo rows x 3 x shifts) once,
+// applying it to each of the 1..4 (tap, row) pairs it serves (216 MFMAs); each
+// block adds its partial sums to dW with one FP32 atomic per element at the end.  Needs cin % 32 == cout % 32 == 0,
 // W % 32, H % 4 and D % 2 == 0.
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 constexpr int kWgDy = 256 * 4;  // 16-byte chunks of the dY tile (256 voxels x 32 channels)
+
+// Byte offset, from the lane's own, of the input fragment of step i of the
+// weight gradient's loop: halo row i / 3 (plane-major), x shift i % 3.
+__host__ __device__ constexpr unsigned wg_b_off(int i) {
+  return (unsigned)(((i / 3) * kHaloX + i % 3) * (32 * 2));
+}
 
 __device__ __forceinline__ bf16x8 tr_frag(const unsigned char* lds, unsigned off_lo, unsigned off_hi) {
   typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
@@ -1058,20 +1065,42 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_tr_kernel(Wgrad g, int tiles
     dyt[t + 512] = d2;
     dyt[t + 768] = d3;
     __syncthreads();
-#pragma unroll 1
-    for (int kc = 0; kc < 8; ++kc) {  // tile row: plane kc / 4, row kc % 4
-      const int pz = kc >> 2, py = kc & 3;
-      const unsigned ao = a_lane + (unsigned)kc * (32 * 32 * 2);
-      const bf16x8 af = tr_frag(db, ao, ao + 4 * 32 * 2);
+    // the tile's 8 dY fragments (row kc: plane kc / 4, row kc % 4) stay in
+    // registers; each input fragment (halo row r = (hz, hy), x shift dx) is
+    // read once, two steps ahead, and feeds tap (hz - pz, hy - py, dx) of every
+    // tile row (pz, py) it lies under
+    bf16x8 af[8];
 #pragma unroll
-      for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        const unsigned bo =
-            b_lane + (unsigned)((((pz + dz) * kHaloRows + py + dy) * kHaloX + dx) * (32 * 2));
-        const bf16x8 bfr = tr_frag(hb, bo, bo + 4 * 32 * 2);
-        acc[tap] = mfma(af, bfr, acc[tap]);
-      }
+    for (int kc = 0; kc < 8; ++kc) {
+      const unsigned ao = a_lane + (unsigned)kc * (32 * 32 * 2);
+      af[kc] = tr_frag(db, ao, ao + 4 * 32 * 2);
     }
+    constexpr int kSteps = kHaloPlanes * kHaloRows * 3, kAhead = 4;
+    bf16x8 br[kAhead + 1];
+#pragma unroll
+    for (int i = 0; i < kAhead; ++i) br[i] = tr_frag(hb, b_lane + wg_b_off(i), b_lane + wg_b_off(i) + 256);
+#pragma unroll
+    for (int hz = 0; hz < kHaloPlanes; ++hz)
+#pragma unroll
+      for (int hy = 0; hy < kHaloRows; ++hy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const int i = (hz * kHaloRows + hy) * 3 + dx;
+          if (i + kAhead < kSteps) {
+            const unsigned bo = b_lane + wg_b_off(i + kAhead);
+            br[(i + kAhead) % (kAhead + 1)] = tr_frag(hb, bo, bo + 4 * 32 * 2);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+              const int pz = hz - dz, py = hy - dy;
+              if (pz < 0 || pz > 1 || py < 0 || py > 3) continue;
+              const int tap = (dz * 3 + dy) * 3 + dx;
+              acc[tap] = mfma(af[pz * 4 + py], br[i % (kAhead + 1)], acc[tap]);
+            }
+        }
   }
   // lane (sl, kq) holds co = co0 + 16 ct + 4 kq + i, ci = ci0 + 16 it + sl
 #pragma unroll
