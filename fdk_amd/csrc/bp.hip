@@ -792,21 +792,32 @@ struct WsPre {
 // Columns outside the row read other samples of the view or (negative or past
 // the view) zero through the buffer range check; only columns of valid taps
 // are ever used.
+// Branch-free: every offset is computed before the first load, and quads past
+// the strip take an offset past the view (zeros, never used), so the loads of
+// one view issue back to back and no register reuse has to wait on them.
 template <int DT>
 __device__ __forceinline__ void ws_issue(WsPre<DT>& pre, __amdgpu_buffer_rsrc_t rv, int W, int iu,
                                          int c0, int quads, int lane) {
   const int es = DT == FDK_F32 ? 4 : 2;
   const int rowa = iu * W * es, rowb = rowa + W * es;
+  constexpr int kOut = (int)0xFFFFFFF0u;  // past any view's buffer range
+  int ta[kWsQuads], tb[kWsQuads], ma[kWsQuads], mb[kWsQuads];
 #pragma unroll
   for (int k = 0; k < kWsQuads; ++k) {
     const int q = lane + kWave * k;
-    if (q < quads) {
-      const int ct = (c0 + 4 * q) * es, cm = (W - 4 - c0 - 4 * q) * es;
-      pre.ta[k] = Quad<DT>::load(rv, rowa + ct);
-      pre.tb[k] = Quad<DT>::load(rv, rowb + ct);
-      pre.ma[k] = Quad<DT>::load(rv, rowa + cm);
-      pre.mb[k] = Quad<DT>::load(rv, rowb + cm);
-    }
+    const bool ok = q < quads;
+    const int ct = (c0 + 4 * q) * es, cm = (W - 4 - c0 - 4 * q) * es;
+    ta[k] = ok ? rowa + ct : kOut;
+    tb[k] = ok ? rowb + ct : kOut;
+    ma[k] = ok ? rowa + cm : kOut;
+    mb[k] = ok ? rowb + cm : kOut;
+  }
+#pragma unroll
+  for (int k = 0; k < kWsQuads; ++k) {
+    pre.ta[k] = Quad<DT>::load(rv, ta[k]);
+    pre.tb[k] = Quad<DT>::load(rv, tb[k]);
+    pre.ma[k] = Quad<DT>::load(rv, ma[k]);
+    pre.mb[k] = Quad<DT>::load(rv, mb[k]);
   }
 }
 
