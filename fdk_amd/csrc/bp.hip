@@ -758,6 +758,11 @@ __global__ __launch_bounds__(256) void bp_lds_pf_kernel(BpArgs a, BpMats M, BpSc
 // Per voxel pair: v, floor, fraction, 1-t, the LDS address (and the validity
 // test where the run crosses the detector edge) and four FMAs.
 constexpr int kWsQuads = 3;  // strip quads per lane and run (the host checks CT against it)
+// Quad slots a lane stages at a given ZPT: a 256-voxel run's strip fits 2 x 64
+// quads, and an always-empty third slot would still cost its (range-checked,
+// zero) loads and offsets every view.
+template <int ZPT>
+constexpr int ws_quads() { return ZPT == 4 ? 2 : kWsQuads; }
 
 // Per-view values of one column, kept in lane s for view s.
 struct WaveView {
@@ -784,7 +789,7 @@ __device__ __forceinline__ float4 ws_combine(typename Quad<DT>::raw A, typename 
 
 template <int DT>
 struct WsPre {
-  typename Quad<DT>::raw ta[kWsQuads], tb[kWsQuads], ma[kWsQuads], mb[kWsQuads];
+  typename Quad<DT>::raw ta[kWsQuads], tb[kWsQuads], ma[kWsQuads], mb[kWsQuads];  // NQ <= kWsQuads used
 };
 
 // Strip loads of one view: quad q of the top run reads columns c0+4q..c0+4q+3
@@ -795,15 +800,15 @@ struct WsPre {
 // Branch-free: every offset is computed before the first load, and quads past
 // the strip take an offset past the view (zeros, never used), so the loads of
 // one view issue back to back and no register reuse has to wait on them.
-template <int DT>
+template <int DT, int NQ = kWsQuads>
 __device__ __forceinline__ void ws_issue(WsPre<DT>& pre, __amdgpu_buffer_rsrc_t rv, int W, int iu,
                                          int c0, int quads, int lane) {
   const int es = DT == FDK_F32 ? 4 : 2;
   const int rowa = iu * W * es, rowb = rowa + W * es;
   constexpr int kOut = (int)0xFFFFFFF0u;  // past any view's buffer range
-  int ta[kWsQuads], tb[kWsQuads], ma[kWsQuads], mb[kWsQuads];
+  int ta[NQ], tb[NQ], ma[NQ], mb[NQ];
 #pragma unroll
-  for (int k = 0; k < kWsQuads; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     const int q = lane + kWave * k;
     const bool ok = q < quads;
     const int ct = (c0 + 4 * q) * es, cm = (W - 4 - c0 - 4 * q) * es;
@@ -813,7 +818,7 @@ __device__ __forceinline__ void ws_issue(WsPre<DT>& pre, __amdgpu_buffer_rsrc_t 
     mb[k] = ok ? rowb + cm : kOut;
   }
 #pragma unroll
-  for (int k = 0; k < kWsQuads; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     pre.ta[k] = Quad<DT>::load(rv, ta[k]);
     pre.tb[k] = Quad<DT>::load(rv, tb[k]);
     pre.ma[k] = Quad<DT>::load(rv, ma[k]);
@@ -821,11 +826,11 @@ __device__ __forceinline__ void ws_issue(WsPre<DT>& pre, __amdgpu_buffer_rsrc_t 
   }
 }
 
-template <int DT>
+template <int DT, int NQ = kWsQuads>
 __device__ __forceinline__ void ws_commit(const WsPre<DT>& pre, float2* strip, float w0, float w1,
                                           float zof, int quads, int lane) {
 #pragma unroll
-  for (int k = 0; k < kWsQuads; ++k) {
+  for (int k = 0; k < NQ; ++k) {
     const int q = lane + kWave * k;
     if (q < quads) {
       const float4 t = ws_combine<DT>(pre.ta[k], pre.tb[k], w0, w1, zof);
@@ -913,15 +918,15 @@ __global__ __launch_bounds__(256) void bp_wave_kernel(BpArgs a, BpMats M, BpScal
     wv.c0 = max(ilo, 0) & ~3;
     const int c1 = min(ihi + 1, a.W - 1);
     wv.quads = any ? (c1 - wv.c0 + 4) >> 2 : 0;
-    const bool fits = 4 * wv.quads <= CT - 2 && wv.quads <= kWave * kWsQuads;
+    const bool fits = 4 * wv.quads <= CT - 2 && wv.quads <= kWave * ws_quads<ZPT>();
     wv.flags = (any ? 1 : 0) | (all ? 2 : 0) | (fits ? 4 : 0);
   }
 
   WsPre<DT> pre;
   int f_n = lane_i(wv.flags, 0);
   if ((f_n & 5) == 5)
-    ws_issue<DT>(pre, view_rsrc(a, 0), a.W, lane_i(wv.iu, 0), lane_i(wv.c0, 0),
-                 lane_i(wv.quads, 0), lane);
+    ws_issue<DT, ws_quads<ZPT>()>(pre, view_rsrc(a, 0), a.W, lane_i(wv.iu, 0), lane_i(wv.c0, 0),
+                                  lane_i(wv.quads, 0), lane);
   for (int s = 0; s < a.nproj; ++s) {
     const int f = f_n;
     const float Vb = lane_f(wv.Vb, s), dV = lane_f(wv.dV, s);
@@ -934,13 +939,13 @@ __global__ __launch_bounds__(256) void bp_wave_kernel(BpArgs a, BpMats M, BpScal
       zof = (w0 + w1) * so.y;
     }
     if ((f & 5) == 5)
-      ws_commit<DT>(pre, strip, DT == FDK_F32 ? w0 : w0 * so.x, DT == FDK_F32 ? w1 : w1 * so.x, zof,
-                    quads, lane);
+      ws_commit<DT, ws_quads<ZPT>()>(pre, strip, DT == FDK_F32 ? w0 : w0 * so.x,
+                                     DT == FDK_F32 ? w1 : w1 * so.x, zof, quads, lane);
     if (s + 1 < a.nproj) {
       f_n = lane_i(wv.flags, s + 1);
       if ((f_n & 5) == 5)
-        ws_issue<DT>(pre, view_rsrc(a, s + 1), a.W, lane_i(wv.iu, s + 1), lane_i(wv.c0, s + 1),
-                     lane_i(wv.quads, s + 1), lane);
+        ws_issue<DT, ws_quads<ZPT>()>(pre, view_rsrc(a, s + 1), a.W, lane_i(wv.iu, s + 1),
+                                      lane_i(wv.c0, s + 1), lane_i(wv.quads, s + 1), lane);
     }
     if (!(f & 1)) continue;
     if (f & 4) {
@@ -1255,7 +1260,7 @@ bool launch_wave(fdk_bp* h, const BpArgs& a, const BpMats& M, const BpScale& S, 
     return false;
   if (!mirror_symmetric(a, M)) return false;
   const int CT = wave_strip_ct(a, M, ZPT);
-  if (CT <= 0 || (CT - 2) / 4 > kWave * kWsQuads) return false;
+  if (CT <= 0 || (CT - 2) / 4 > kWave * ws_quads<ZPT>()) return false;
   const size_t lds = (size_t)4 * CT * sizeof(float2);
   if (lds > 64 * 1024) return false;
   dim3 grid(nblk, (a.zext + kWave * ZPT - 1) / (kWave * ZPT));

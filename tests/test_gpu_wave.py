@@ -3,8 +3,8 @@ paths, against the oracle's strict-bilinear backprojection through the C ABI:
 
 * a tall volume whose z-runs leave the detector (runs partly inside: the
   checked tap loop; runs wholly outside: skipped views);
-* a deep slab pair (512-voxel runs, ZPT 8) and a shallow one (ZPT 4, runs
-  padded past the slab);
+* a deep slab pair (512-voxel runs, ZPT 8) and shallow ones (ZPT 4, two strip
+  quads per lane, runs padded past the slab; one with an odd column count);
 * a detector with a v offset, whose matrices are not mirror-symmetric: the
   block-strip kernels run instead, with the same results;
 * 16-bit views dequantized in the strip combine, against the FP32 path on the
@@ -80,6 +80,9 @@ CASES = {
     # 64-wide detector rows (v) for a 200-deep volume: most runs cross the edge
     "tall": (dict(nx=40, ny=36, nz=200, dx=2.0, dy=2.0, dz=2.0, nu=72, nv=64, du=1.8, dv=1.8,
                   nproj=40, volume_position_angle_rad=0.2), "mirror"),
+    # 100-deep slabs (ZPT 4) with an odd column count along y
+    "odd": (dict(nx=40, ny=37, nz=200, dx=2.0, dy=2.0, dz=2.0, nu=72, nv=64, du=1.8, dv=1.8,
+                 nproj=33, volume_position_angle_rad=0.3), "mirror"),
     # 1024 deep: 512-voxel runs (ZPT 8)
     "deep": (dict(nx=32, ny=32, nz=1024, dx=0.5, dy=0.5, dz=0.5, nu=48, nv=800, du=0.75, dv=0.75,
                   nproj=36, volume_position_angle_rad=0.0), "mirror"),
