@@ -1134,6 +1134,61 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_c1_kernel(Wgrad a) {
   for (int t = 0; t < 27; ++t) atomicAdd(&a.dw[co * 32 + t], acc[t]);
 }
 
+// The same weight gradient on 2-plane x 4-row x 64-voxel tiles: the block
+// stages the tile's input halo (4 x 6 x 66 values, zero outside the volume) in
+// LDS as FP32; thread (co = t % 32, tile row t / 32) walks its row's 64 voxels
+// with the 3 x 3 x 3 input window in registers (9 new values per voxel, the
+// unrolled walk renames the rest) and its 27 sums, one dY value per voxel
+// (32 channels of one voxel = 64 contiguous bytes per 32 lanes).  Each block
+// adds its sums with 27 x 32 FP32 atomics at the end.  W % 64, H % 4, D % 2.
+constexpr int kC1X = 64;  // tile row length
+__global__ __launch_bounds__(256) void conv3d_wgrad_c1t_kernel(Wgrad a, int tiles_x, int tiles_y,
+                                                               int ntiles) {
+  __shared__ float xs[4][6][kC1X + 2];
+  const int t = threadIdx.x, co = t & 31, r = t >> 5, pz = r >> 2, py = r & 3;
+  float acc[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) acc[k] = 0.f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % tiles_x, rr = tile / tiles_x;
+    const int y0 = (rr % tiles_y) * 4, zb = (rr / tiles_y) * 2, x0 = tx * kC1X;
+    const int n = zb / a.d, z0 = zb - n * a.d;
+    __syncthreads();  // the previous tile's window reads are done
+    for (int i = t; i < 4 * 6 * (kC1X + 2); i += 256) {
+      const int hz = i / (6 * (kC1X + 2)), hy = (i / (kC1X + 2)) % 6, hx = i % (kC1X + 2);
+      const int zz = z0 - 1 + hz, yy = y0 - 1 + hy, xx = x0 - 1 + hx;
+      const bool in = zz >= 0 && zz < a.d && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+      xs[hz][hy][hx] = in ? bf2f(a.x[(((long long)n * a.d + zz) * a.h + yy) * a.w + xx]) : 0.f;
+    }
+    __syncthreads();
+    const u16* dyr = a.dy + (((long long)(zb + pz) * a.h + y0 + py) * a.w + x0) * 32 + co;
+    float win[3][3][3];  // win[dz][dy][dx] = X at (pz + dz - 1, py + dy - 1, x + dx - 1)
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        win[dz][dy][1] = xs[pz + dz][py + dy][0];
+        win[dz][dy][2] = xs[pz + dz][py + dy][1];
+      }
+#pragma unroll
+    for (int x = 0; x < kC1X; ++x) {
+      const float g = bf2f(dyr[(long long)x * 32]);
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          win[dz][dy][0] = win[dz][dy][1];
+          win[dz][dy][1] = win[dz][dy][2];
+          win[dz][dy][2] = xs[pz + dz][py + dy][x + 2];
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx) acc[(dz * 3 + dy) * 3 + dx] = fmaf(g, win[dz][dy][dx], acc[(dz * 3 + dy) * 3 + dx]);
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 27; ++k) atomicAdd(&a.dw[co * 32 + k], acc[k]);
+}
+
 // dW of a [27][cout][cin] forward weight -> backward-data weight
 // Wb[t][ci][co] = W[26 - t][co][ci] (spatial flip, channel transpose).
 __global__ void flip_weights_kernel(const u16* w, u16* wb, int cin, int cout) {
@@ -1156,9 +1211,27 @@ __global__ void bias_grad_kernel(const u16* dy, float* db, long long nvox, int c
 
 // Adjoint of the activation at output y (mode 1 ReLU, 2 leaky ReLU: y > 0
 // exactly where the input was).
+__device__ __forceinline__ u16 relu_bwd1(u16 dy, u16 y, int mode) {
+  return (bf2f(y) > 0.f) ? dy : mode == 2 ? f2bf(kLeak * bf2f(dy)) : (u16)0;
+}
+
 __global__ void relu_bwd_kernel(const u16* dy, const u16* y, u16* dz, long long n, int mode) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    dz[i] = (bf2f(y[i]) > 0.f) ? dy[i] : mode == 2 ? f2bf(kLeak * bf2f(dy[i])) : (u16)0;
+    dz[i] = relu_bwd1(dy[i], y[i], mode);
+}
+
+// The same, 8 values per 16-byte vector (n % 8 == 0, 16-byte aligned).
+__global__ void relu_bwd_v8_kernel(const uint4* dy, const uint4* y, uint4* dz, long long n8, int mode) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n8; i += (long long)gridDim.x * 256) {
+    const uint4 g = dy[i], v = y[i];
+    const unsigned gw[4] = {g.x, g.y, g.z, g.w}, vw[4] = {v.x, v.y, v.z, v.w};
+    unsigned o[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      o[e] = (unsigned)relu_bwd1((u16)(gw[e] & 0xFFFF), (u16)(vw[e] & 0xFFFF), mode) |
+             ((unsigned)relu_bwd1((u16)(gw[e] >> 16), (u16)(vw[e] >> 16), mode) << 16);
+    dz[i] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
 }
 
 // 2x2x2 average pooling (and its adjoint) on [n][d][h][w][c].
@@ -1350,6 +1423,73 @@ __global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w,
   __syncthreads();
   for (int i = threadIdx.x; i < c; i += 256) atomicAdd(&dw[i], red[i]);
   if (threadIdx.x == 0) atomicAdd(db, red[c]);
+}
+
+// head_bwd_kernel for C channels with 16-byte vectors: one voxel per thread,
+// its C BF16 features as C / 8 16-byte loads and its feature gradients as C / 8
+// 16-byte stores; the weight-gradient sums go through LDS atomics per block
+// and one global atomic per channel and block.
+template <int C>
+__global__ __launch_bounds__(256) void head_bwd_v8_kernel(const float* dout, const uint4* h,
+                                                          const float* w, uint4* dh, float* dw,
+                                                          float* db, long long nvox) {
+  __shared__ float red[C + 1];
+  for (int i = threadIdx.x; i < C + 1; i += 256) red[i] = 0.f;
+  __syncthreads();
+  float acc[C];
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) acc[ch] = 0.f;
+  float accb = 0.f;
+  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long long)gridDim.x * 256) {
+    const float g = dout[v];
+    accb += g;
+#pragma unroll
+    for (int q = 0; q < C / 8; ++q) {
+      const uint4 hv = h[v * (C / 8) + q];
+      const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
+      unsigned o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ch = 8 * q + 2 * e;
+        acc[ch] = fmaf(g, __uint_as_float(hw[e] << 16), acc[ch]);
+        acc[ch + 1] = fmaf(g, __uint_as_float(hw[e] & 0xFFFF0000u), acc[ch + 1]);
+        o[e] = (unsigned)f2bf(g * w[ch]) | ((unsigned)f2bf(g * w[ch + 1]) << 16);
+      }
+      dh[v * (C / 8) + q] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < C; ++ch) atomicAdd(&red[ch], acc[ch]);
+  atomicAdd(&red[C], accb);
+  __syncthreads();
+  for (int i = threadIdx.x; i < C; i += 256) atomicAdd(&dw[i], red[i]);
+  if (threadIdx.x == 0) atomicAdd(db, red[C]);
+}
+
+// bias_grad_kernel for C % 8 == 0 channels with 16-byte loads: thread t sums
+// channels 8 (t % (C / 8)) .. + 7 over every (256 / (C / 8))-th voxel of the
+// block's share; partial sums meet in LDS, then one atomic per channel.
+template <int C>
+__global__ __launch_bounds__(256) void bias_grad_v8_kernel(const uint4* dy, float* db, long long nvox) {
+  constexpr int G = C / 8, L = 256 / G;  // channel groups, voxel lanes
+  __shared__ float red[C];
+  for (int i = threadIdx.x; i < C; i += 256) red[i] = 0.f;
+  __syncthreads();
+  const int g = threadIdx.x % G, l = threadIdx.x / G;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (long long v = (long long)blockIdx.x * L + l; v < nvox; v += (long long)gridDim.x * L) {
+    const uint4 q = dy[v * G + g];
+    const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      s[2 * e] += __uint_as_float(u[e] << 16);
+      s[2 * e + 1] += __uint_as_float(u[e] & 0xFFFF0000u);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) atomicAdd(&red[8 * g + e], s[e]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < C; i += 256) atomicAdd(&db[i], red[i]);
 }
 
 // Space-to-depth at the network's ends, block edge F (2 or 4).  An FP32 volume
@@ -1703,11 +1843,11 @@ static int conv_kernel_choice() {
 
 static bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
-// Shapes and alignments conv3d_fwd4_kernel takes (cout 32 or a multiple of 64, whole
+// Shapes and alignments the LDS-halo kernels take (cout a multiple of 32, whole
 // 32-channel chunks, even depth, 16-byte vectors, 32-bit staging offsets).
 static bool lean_ok(const Conv& a) {
   const int cx = a.cin - a.cup;
-  return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && (a.cout == 32 || a.cout % 64 == 0) &&
+  return a.cin % 32 == 0 && a.cup % 32 == 0 && a.d % 2 == 0 && a.cout % 32 == 0 &&
          aligned16(a.x) && aligned16(a.wt) && aligned16(a.y) && (!a.bias || aligned16(a.bias)) &&
          (!a.resid || aligned16(a.resid)) && (!a.up || aligned16(a.up)) &&
          ((4LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8) &&
@@ -1727,20 +1867,21 @@ static int launch_fwd5_pass(const Conv& a, hipStream_t st) {
   return FDK_OK;
 }
 
-// conv3d_fwd5_kernel where its tile fits: cout 32 with d % 4 == 0 (4-plane
-// tiles) or cout % 64 == 0 (2-plane tiles, 64-channel passes); h % 4 == 0; its
-// halo resource ((TP + 2) planes + 2 rows) below 2^32 - 16 bytes.
+// conv3d_fwd5_kernel where its tile fits: 64-channel passes on 2-plane tiles
+// for the first 64 floor(cout / 64) channels, a 32-channel pass on 4-plane tiles
+// (d % 4 == 0) for the rest (cout 32, 96, ...); h % 4 == 0; its halo resource
+// ((TP + 2) planes + 2 rows) below 2^32 - 16 bytes.
 static bool fwd5_ok(const Conv& a) {
   const int cx = a.cin - a.cup;
-  return conv_kernel_choice() >= 5 && ((a.cout == 32 && a.d % 4 == 0) || a.cout % 64 == 0) &&
+  return conv_kernel_choice() >= 5 && a.cout % 32 == 0 && (a.cout % 64 == 0 || a.d % 4 == 0) &&
          a.h % 4 == 0 && (6LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8;
 }
 
 static int launch_fwd5(const Conv& a, hipStream_t st) {
-  if (a.cout == 32) return launch_fwd5_pass<4, true>(a, st);
   Conv p = a;
-  for (p.co0 = 0; p.co0 < a.cout; p.co0 += 64)
+  for (p.co0 = 0; p.co0 + 64 <= a.cout; p.co0 += 64)
     if (int r = launch_fwd5_pass<2, true>(p, st)) return r;
+  if (p.co0 < a.cout) return launch_fwd5_pass<4, true>(p, st);  // the last 32 channels
   return FDK_OK;
 }
 
@@ -1749,15 +1890,14 @@ static int launch_fwd4(const Conv& a, hipStream_t st) {
   const int tiles_x = a.w / 32, tiles_y = a.h / 4, ntiles = tiles_x * tiles_y * (a.n * a.d / 2);
   const int tpb = 2;
   const dim3 g4((ntiles + tpb - 1) / tpb);
-  if (a.cout == 32) {
-    hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, a, tiles_x, tiles_y, ntiles, tpb);
-  } else {
-    // 64 output channels per pass; cout 128 and 192 re-stage the halo per pass,
-    // which costs less than the wider accumulator set would in registers
-    Conv p = a;
-    for (p.co0 = 0; p.co0 < a.cout; p.co0 += 64)
-      hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, p, tiles_x, tiles_y, ntiles, tpb);
-  }
+  // 64 output channels per pass (cout 128 and 192 re-stage the halo per pass,
+  // which costs less than the wider accumulator set would in registers), then
+  // a 32-channel pass for cout 32, 96, ...
+  Conv p = a;
+  for (p.co0 = 0; p.co0 + 64 <= a.cout; p.co0 += 64)
+    hipLaunchKernelGGL((conv3d_fwd4_kernel<4, false>), g4, dim3(256), 0, st, p, tiles_x, tiles_y, ntiles, tpb);
+  if (p.co0 < a.cout)
+    hipLaunchKernelGGL((conv3d_fwd4_kernel<2, true>), g4, dim3(256), 0, st, p, tiles_x, tiles_y, ntiles, tpb);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -1828,7 +1968,7 @@ extern "C" int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, con
   Conv a{static_cast<const u16*>(x), static_cast<const u16*>(w), bias, resid, y, n, d, h, wd,
          cin, cout, relu, out_f32, static_cast<const u16*>(up), cup};
   if (!lean_ok(a))
-    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_fwd_cat: cout %d (32 or a multiple of 64), 16-byte aligned "
+    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_fwd_cat: cout %d (a multiple of 32), 16-byte aligned "
                      "tensors and 32-bit halo offsets required", cout);
   return launch_fwd4(a, static_cast<hipStream_t>(stream));
 }
@@ -1841,7 +1981,13 @@ extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int
   Wgrad a{static_cast<const u16*>(x), static_cast<const u16*>(dy), dw, n, d, h, wd, cin, cout};
   if (cin == 1) {
     FDK_REQUIRE(cout == 32, "the cin == 1 weight gradient takes cout == 32");
-    hipLaunchKernelGGL(conv3d_wgrad_c1_kernel, dim3(2048), dim3(256), 0, st, a);
+    if (wd % kC1X == 0 && h % 4 == 0 && d % 2 == 0 && aligned16(dy)) {
+      const int tiles_x = wd / kC1X, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
+      hipLaunchKernelGGL(conv3d_wgrad_c1t_kernel, dim3(std::min(ntiles, 1024)), dim3(256), 0, st, a,
+                         tiles_x, tiles_y, ntiles);
+    } else {
+      hipLaunchKernelGGL(conv3d_wgrad_c1_kernel, dim3(2048), dim3(256), 0, st, a);
+    }
     FDK_HIP(hipGetLastError());
     return FDK_OK;
   }
@@ -1882,8 +2028,18 @@ extern "C" int fdk_nn_flip_weights(const void* w, void* wb, int cin, int cout, v
 
 extern "C" int fdk_nn_bias_grad(const void* dy, float* db, int64_t nvox, int c, void* stream) {
   FDK_REQUIRE(dy && db && c > 0 && c <= 256 && 256 % c == 0, "bias_grad needs c dividing 256");
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(1024), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const u16*>(dy), db, (long long)nvox, c);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(1024), b(256);
+  const uint4* v = static_cast<const uint4*>(dy);
+  if (!aligned16(dy)) c = -c;  // the scalar kernel below
+  switch (c) {
+    case 32: hipLaunchKernelGGL(bias_grad_v8_kernel<32>, g, b, 0, st, v, db, (long long)nvox); break;
+    case 64: hipLaunchKernelGGL(bias_grad_v8_kernel<64>, g, b, 0, st, v, db, (long long)nvox); break;
+    case 128: hipLaunchKernelGGL(bias_grad_v8_kernel<128>, g, b, 0, st, v, db, (long long)nvox); break;
+    default:
+      hipLaunchKernelGGL(bias_grad_kernel, g, b, 0, st, static_cast<const u16*>(dy), db, (long long)nvox,
+                         c < 0 ? -c : c);
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -1892,9 +2048,14 @@ extern "C" int fdk_nn_relu_bwd(const void* dy, const void* y, void* dz, int64_t 
                                void* stream) {
   FDK_REQUIRE(dy && y && dz, "NULL pointer");
   FDK_REQUIRE(mode == 1 || mode == 2, "activation mode must be 1 (ReLU) or 2 (leaky ReLU)");
-  hipLaunchKernelGGL(relu_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const u16*>(dy), static_cast<const u16*>(y), static_cast<u16*>(dz), (long long)n,
-                     mode);
+  if (n % 8 == 0 && aligned16(dy) && aligned16(y) && aligned16(dz))
+    hipLaunchKernelGGL(relu_bwd_v8_kernel, dim3(blocks_for(n / 8)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const uint4*>(dy),
+                       static_cast<const uint4*>(y), static_cast<uint4*>(dz), (long long)(n / 8), mode);
+  else
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(dy), static_cast<const u16*>(y), static_cast<u16*>(dz),
+                       (long long)n, mode);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -1983,9 +2144,13 @@ extern "C" int fdk_nn_head(const void* h, const float* w, const float* b, const 
 extern "C" int fdk_nn_head_bwd(const float* dout, const void* h, const float* w, void* dh,
                                float* dw, float* db, int64_t nvox, int c, void* stream) {
   FDK_REQUIRE(dout && h && w && dh && dw && db && c <= 63, "bad head arguments");
-  hipLaunchKernelGGL(head_bwd_kernel, dim3(std::min<long long>(1024, blocks_for(nvox))), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), dout, static_cast<const u16*>(h), w,
-                     static_cast<u16*>(dh), dw, db, (long long)nvox, c);
+  const dim3 g(std::min<long long>(1024, blocks_for(nvox)));
+  if (c == 32 && aligned16(h) && aligned16(dh))
+    hipLaunchKernelGGL(head_bwd_v8_kernel<32>, g, dim3(256), 0, static_cast<hipStream_t>(stream), dout,
+                       static_cast<const uint4*>(h), w, static_cast<uint4*>(dh), dw, db, (long long)nvox);
+  else
+    hipLaunchKernelGGL(head_bwd_kernel, g, dim3(256), 0, static_cast<hipStream_t>(stream), dout,
+                       static_cast<const u16*>(h), w, static_cast<u16*>(dh), dw, db, (long long)nvox, c);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

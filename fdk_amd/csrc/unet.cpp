@@ -229,7 +229,7 @@ int conv(const Layer& l, int act, const void* x, void* y, int n, int d, int h, i
 int conv_cat(Pass& P, const Layer& l, int act, const void* up, int cup, const void* skip, void* y,
              int n, int d, int h, int w) {
   const int cskip = l.cin - cup;
-  if ((l.cout == 32 || l.cout % 64 == 0) && cup % 32 == 0 && cskip > 0 && l.cin % 32 == 0)
+  if (l.cout % 32 == 0 && cup % 32 == 0 && cskip > 0 && l.cin % 32 == 0)
     return fdk_nn_conv3d_fwd_cat(up, cup, skip, l.w, l.b, nullptr, y, n, d, h, w, l.cin, l.cout,
                                  act, 0, P.st);
   void* cat = nullptr;

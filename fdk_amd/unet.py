@@ -234,7 +234,7 @@ class Conv3:
 
     def cat_ok(self, cup):
         """Whether forward_cat serves this layer (the fused decoder-input kernel)."""
-        return ((self.cout == 32 or self.cout % 64 == 0) and self.cin % 32 == 0 and cup % 32 == 0
+        return (self.cout % 32 == 0 and self.cin % 32 == 0 and cup % 32 == 0
                 and 0 < cup < self.cin)
 
     def forward_cat(self, up, cup, x, shape):

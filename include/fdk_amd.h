@@ -316,7 +316,7 @@ int fdk_nn_conv3d_fwd(const void* x, const void* w, const float* bias, const flo
  * [nearest 2x upsampling of up, x] (up: [n][d/2][h/2][w/2][cup], x:
  * [n][d][h][w][cin - cup]), read from the two tensors without forming the
  * concatenation.  Bit-identical to fdk_nn_conv3d_fwd on the formed input.
- * Needs cout 32 or a multiple of 64, cup and cin multiples of 32, even d, 16-byte aligned
+ * Needs cout, cup and cin multiples of 32, even d, 16-byte aligned
  * tensors; FDK_ERR_UNSUPPORTED otherwise. */
 int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, const void* w, const float* bias,
                           const float* resid, void* y, int n, int d, int h, int w_, int cin,

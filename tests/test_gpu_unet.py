@@ -69,11 +69,12 @@ def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
     """Even-depth shapes, which take the LDS-halo kernels (two tiles per block;
     cout 128 as two 64-channel passes): interior and border tiles, both batch
     items, one to three 32-channel chunks, and an odd tile count (3 tiles: the
-    last block has one).  Depth 6 and 2 run the 2-plane tiles; at cout 32,
-    depth 8, 4 and 12 run the 4-plane x 4-row row-reuse tiles."""
+    last block has one).  Depth 6 and 2 run the 2-plane tiles; depth 8, 4 and
+    12 the row-reuse tiles (4 planes at cout 32, 2 planes per 64-channel pass;
+    cout 96 as a 64-channel pass and a 32-channel pass)."""
     torch = torch_cuda
     rng = np.random.default_rng(cin + shape[0])
-    for cout in (32, 64, 128):
+    for cout in (32, 64, 96, 128):
         x = R.bf16_round(rng.standard_normal(shape + (cin,)))
         w = R.bf16_round(rng.standard_normal((27, cout, cin)) / np.sqrt(27 * cin))
         b = rng.standard_normal(cout).astype(np.float32)
@@ -94,7 +95,9 @@ def test_conv3d_forward_staged_tiles(torch_cuda, cin, shape):
 
 @pytest.mark.parametrize("cup,cskip,cout,shape", [(64, 32, 32, (2, 8, 12, 96)),
                                                   (128, 64, 64, (1, 8, 12, 96)),
-                                                  (64, 32, 32, (1, 2, 4, 32))])
+                                                  (64, 32, 32, (1, 2, 4, 32)),
+                                                  (64, 32, 96, (1, 8, 12, 64)),
+                                                  (64, 32, 96, (1, 6, 8, 32))])
 def test_conv3d_decoder_input_fused(torch_cuda, cup, cskip, cout, shape):
     """conv3d of [nearest 2x upsampling of a coarse tensor, skip] read from the two
     tensors (fdk_nn_conv3d_fwd_cat) is bit-identical to upsample + channel copy +
@@ -198,10 +201,12 @@ def test_conv3d_weight_gradient_tiled(torch_cuda, cin, cout, shape):
     assert rel(dw.cpu().numpy(), 2 * R.conv3d_dw(x, dy)) < 1e-5
 
 
-def test_conv3d_weight_gradient_single_input_channel(torch_cuda):
+@pytest.mark.parametrize("shape", [(1, 4, 8, 64), (2, 6, 12, 128), (1, 3, 8, 32)])
+def test_conv3d_weight_gradient_single_input_channel(torch_cuda, shape):
+    """The tiled form (W % 64, H % 4, D % 2: border and interior tiles, both
+    batch items, several tiles per block) and the per-voxel form (odd depth)."""
     torch = torch_cuda
-    rng = np.random.default_rng(9)
-    shape = (1, 4, 8, 64)
+    rng = np.random.default_rng(9 + shape[1])
     x = R.bf16_round(rng.standard_normal(shape + (1,)))
     dy = R.bf16_round(rng.standard_normal(shape + (32,)))
     dw = torch.zeros((32, 32), device="cuda")
@@ -209,6 +214,59 @@ def test_conv3d_weight_gradient_single_input_channel(torch_cuda):
     got = dw.cpu().numpy()
     assert np.abs(got[:, 27:]).max() == 0.0
     assert rel(got[:, :27], R.conv3d_dw(x, dy)[:, :, 0].T) < 1e-5
+
+
+@pytest.mark.parametrize("c", [16, 32, 64, 128])
+def test_bias_gradient(torch_cuda, c):
+    """Per-channel sums of a BF16 gradient (16-byte vector form for 32, 64 and
+    128 channels, the scalar form for 16), accumulated into db."""
+    torch = torch_cuda
+    rng = np.random.default_rng(c)
+    nvox = 3 * 4099
+    dy = R.bf16_round(rng.standard_normal((nvox, c)))
+    db = torch.ones(c, device="cuda")
+    U.bias_grad(bf(torch, dy), db, nvox, c)
+    assert rel(db.cpu().numpy(), 1.0 + dy.sum(0)) < 1e-5
+
+
+@pytest.mark.parametrize("n,mode", [(8 * 5003, 1), (8 * 5003, 2), (5003, 1), (5003, 2)])
+def test_activation_backward(torch_cuda, n, mode):
+    """dz = dy where y > 0, else 0 (ReLU) or 0.01 dy rounded to BF16 (leaky
+    ReLU); the 16-byte vector form (n % 8 == 0) and the scalar form."""
+    torch = torch_cuda
+    rng = np.random.default_rng(n + mode)
+    dy = R.bf16_round(rng.standard_normal(n))
+    y = R.bf16_round(rng.standard_normal(n))
+    y[::7] = 0.0
+    dz = torch.empty(n, dtype=torch.int16, device="cuda")
+    U.relu_bwd(bf(torch, dy), bf(torch, y), dz, mode)
+    leak = R.bf16_round(np.float32(0.01) * dy.astype(np.float32))  # FP32 product, then RNE
+    ref = np.where(y > 0, dy, 0.0 if mode == 1 else leak)
+    assert np.array_equal(from_bf(dz), ref)
+
+
+@pytest.mark.parametrize("c", [32, 24])
+def test_head_backward(torch_cuda, c):
+    """The 1x1 head's adjoint (16-byte vector form at 32 features, the scalar
+    form otherwise): feature gradients dout * w (BF16) and the weight and bias
+    gradients accumulated in FP32."""
+    torch = torch_cuda
+    rng = np.random.default_rng(c + 1)
+    nvox = 2 * 4097
+    h = R.bf16_round(rng.standard_normal((nvox, c)))
+    dout = rng.standard_normal(nvox).astype(np.float32)
+    w = rng.standard_normal(c).astype(np.float32)
+    dh = torch.empty((nvox, c), dtype=torch.int16, device="cuda")
+    dw = torch.zeros(c, device="cuda")
+    db = torch.zeros(1, device="cuda")
+    t = lambda a: torch.as_tensor(a).cuda()
+    d_out, w_d = t(dout), t(w)
+    U.L().fdk_nn_head_bwd(U._p(d_out), U._p(bf(torch, h)), U._p(w_d), U._p(dh), U._p(dw), U._p(db),
+                          nvox, c, U._st())
+    torch.cuda.synchronize()
+    assert rel(from_bf(dh), dout[:, None] * w[None, :]) < 4e-3
+    assert rel(dw.cpu().numpy(), dout @ h) < 1e-5
+    assert abs(float(db.cpu()[0]) - float(dout.sum())) < 1e-4 * np.abs(dout).sum()
 
 
 def test_pool_upsample_copy_head_mse_adam(torch_cuda):
