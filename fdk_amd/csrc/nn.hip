@@ -1295,6 +1295,37 @@ __global__ void avgpool_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, 
   }
 }
 
+// 8 BF16 values of a 16-byte vector as floats, and back (round to nearest even).
+__device__ __forceinline__ void unpack8(const uint4& q, float (&f)[8]) {
+  const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    f[2 * e] = __uint_as_float(u[e] << 16);
+    f[2 * e + 1] = __uint_as_float(u[e] & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  unsigned u[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) u[e] = (unsigned)f2bf(f[2 * e]) | ((unsigned)f2bf(f[2 * e + 1]) << 16);
+  return make_uint4(u[0], u[1], u[2], u[3]);
+}
+
+// avgpool_bwd_kernel with 8 channels per 16-byte vector (the same values):
+// thread (X, 8-channel group) of fine row (blockIdx.z = n d + Z, blockIdx.y = Y).
+__global__ void avgpool_bwd_v8_kernel(const uint4* dy, uint4* dx, int d, int h, int w, int cq) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= w * cq) return;
+  const int X = idx / cq, ch = idx - X * cq;
+  const int Y = blockIdx.y, nz = blockIdx.z, Z = nz % d, nn = nz / d;
+  const long long src = (((long long)(nn * (d / 2) + Z / 2) * (h / 2) + Y / 2) * (w / 2) + X / 2) * cq + ch;
+  float f[8];
+  unpack8(dy[src], f);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) f[e] *= 0.125f;
+  dx[(((long long)nz * h + Y) * w + X) * cq + ch] = pack8(f);
+}
+
 // Nearest 2x upsampling of x ([n][d][h][w][cx]) into channels [c0, c0+cx) of
 // y ([n][2d][2h][2w][cy]); the adjoint sums the 8 children.
 __global__ void upsample_kernel(const u16* x, u16* y, int n, int d, int h, int w, int cx, int cy,
@@ -1326,6 +1357,28 @@ __global__ void upsample_v8_kernel(const uint4* x, uint4* y, int d, int h, int w
   const long long src = (((long long)(nn * d + Z / 2) * h + Y / 2) * w + X / 2) * cq + ch;
   const long long dst = (((long long)nz * H + Y) * Wd + X) * cyq + c0q + ch;
   y[dst] = x[src];
+}
+
+// upsample_bwd_kernel with 8 channels per 16-byte vector (the same sums, in the
+// same order): thread (x, 8-channel group) of coarse row (blockIdx.z = n d + z,
+// blockIdx.y = y).
+__global__ void upsample_bwd_v8_kernel(const uint4* dy, uint4* dx, int d, int h, int w, int cq,
+                                       int cyq, int c0q) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= w * cq) return;
+  const int x = idx / cq, ch = idx - x * cq;
+  const int y = blockIdx.y, nz = blockIdx.z, z = nz % d, nn = nz / d;
+  const int D = 2 * d, H = 2 * h, Wd = 2 * w;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int Z = 2 * z + (q >> 2), Y = 2 * y + ((q >> 1) & 1), X = 2 * x + (q & 1);
+    float f[8];
+    unpack8(dy[(((long long)(nn * D + Z) * H + Y) * Wd + X) * cyq + c0q + ch], f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] += f[e];
+  }
+  dx[(((long long)nz * h + y) * w + x) * cq + ch] = pack8(s);
 }
 
 __global__ void upsample_bwd_kernel(const u16* dy, u16* dx, int n, int d, int h, int w, int cx,
@@ -1983,7 +2036,7 @@ extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int
     FDK_REQUIRE(cout == 32, "the cin == 1 weight gradient takes cout == 32");
     if (wd % kC1X == 0 && h % 4 == 0 && d % 2 == 0 && aligned16(dy)) {
       const int tiles_x = wd / kC1X, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
-      hipLaunchKernelGGL(conv3d_wgrad_c1t_kernel, dim3(std::min(ntiles, 1024)), dim3(256), 0, st, a,
+      hipLaunchKernelGGL(conv3d_wgrad_c1t_kernel, dim3(std::min(ntiles, 256)), dim3(256), 0, st, a,
                          tiles_x, tiles_y, ntiles);
     } else {
       hipLaunchKernelGGL(conv3d_wgrad_c1_kernel, dim3(2048), dim3(256), 0, st, a);
@@ -2071,12 +2124,19 @@ extern "C" int fdk_nn_avgpool2(const void* x, void* y, int n, int d, int h, int 
     dim3 grid(((w / 2) * cq + 255) / 256, h / 2, n * (d / 2));
     hipLaunchKernelGGL(avgpool_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
                        static_cast<uint4*>(y), d, h, w, cq);
-  } else if (!adjoint)
+  } else if (!adjoint) {
     hipLaunchKernelGGL(avgpool_kernel, dim3(blocks_for(full / 8)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, c);
-  else
+  } else if (c % 8 == 0 && (long long)n * d <= 65535 && h <= 65535 &&
+             reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0) {
+    const int cq = c / 8;
+    dim3 grid((w * cq + 255) / 256, h, n * d);
+    hipLaunchKernelGGL(avgpool_bwd_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
+                       static_cast<uint4*>(y), d, h, w, cq);
+  } else {
     hipLaunchKernelGGL(avgpool_bwd_kernel, dim3(blocks_for(full)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, c);
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -2094,12 +2154,18 @@ extern "C" int fdk_nn_upsample2(const void* x, void* y, int n, int d, int h, int
     dim3 grid((2 * w * cq + 255) / 256, 2 * h, n * 2 * d);
     hipLaunchKernelGGL(upsample_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
                        static_cast<uint4*>(y), d, h, w, cq, cy / 8, c0 / 8);
-  } else if (!adjoint)
+  } else if (!adjoint) {
     hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(small * 8)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, cx, cy, c0);
-  else
+  } else if (v8) {
+    const int cq = cx / 8;
+    dim3 grid((w * cq + 255) / 256, h, n * d);
+    hipLaunchKernelGGL(upsample_bwd_v8_kernel, grid, dim3(256), 0, st, static_cast<const uint4*>(x),
+                       static_cast<uint4*>(y), d, h, w, cq, cy / 8, c0 / 8);
+  } else {
     hipLaunchKernelGGL(upsample_bwd_kernel, dim3(blocks_for(small)), dim3(256), 0, st,
                        static_cast<const u16*>(x), static_cast<u16*>(y), n, d, h, w, cx, cy, c0);
+  }
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }
@@ -2300,12 +2366,29 @@ __global__ void add_bf16_kernel(const u16* a, const u16* b, u16* y, long long n)
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     y[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
+
+__global__ void add_bf16_v8_kernel(const uint4* a, const uint4* b, uint4* y, long long n8) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n8; i += (long long)gridDim.x * 256) {
+    float fa[8], fb[8];
+    unpack8(a[i], fa);
+    unpack8(b[i], fb);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) fa[e] += fb[e];
+    y[i] = pack8(fa);
+  }
+}
 }  // namespace
 
 extern "C" int fdk_nn_add(const void* a, const void* b, void* y, int64_t n, void* stream) {
   FDK_REQUIRE(a && b && y, "NULL pointer");
-  hipLaunchKernelGGL(add_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const u16*>(a), static_cast<const u16*>(b), static_cast<u16*>(y), (long long)n);
+  if (n % 8 == 0 && aligned16(a) && aligned16(b) && aligned16(y))
+    hipLaunchKernelGGL(add_bf16_v8_kernel, dim3(blocks_for(n / 8)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const uint4*>(a),
+                       static_cast<const uint4*>(b), static_cast<uint4*>(y), (long long)(n / 8));
+  else
+    hipLaunchKernelGGL(add_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const u16*>(a), static_cast<const u16*>(b), static_cast<u16*>(y),
+                       (long long)n);
   FDK_HIP(hipGetLastError());
   return FDK_OK;
 }

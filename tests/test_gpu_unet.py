@@ -260,13 +260,38 @@ def test_head_backward(torch_cuda, c):
     dw = torch.zeros(c, device="cuda")
     db = torch.zeros(1, device="cuda")
     t = lambda a: torch.as_tensor(a).cuda()
-    d_out, w_d = t(dout), t(w)
-    U.L().fdk_nn_head_bwd(U._p(d_out), U._p(bf(torch, h)), U._p(w_d), U._p(dh), U._p(dw), U._p(db),
+    d_out, w_d, h_d = t(dout), t(w), bf(torch, h)  # alive until the kernel has run
+    U.L().fdk_nn_head_bwd(U._p(d_out), U._p(h_d), U._p(w_d), U._p(dh), U._p(dw), U._p(db),
                           nvox, c, U._st())
     torch.cuda.synchronize()
     assert rel(from_bf(dh), dout[:, None] * w[None, :]) < 4e-3
     assert rel(dw.cpu().numpy(), dout @ h) < 1e-5
     assert abs(float(db.cpu()[0]) - float(dout.sum())) < 1e-4 * np.abs(dout).sum()
+
+
+@pytest.mark.parametrize("c", [32, 12])
+def test_pool_and_upsample_adjoints_and_add(torch_cuda, c):
+    """The adjoints of 2x average pooling and of 2x nearest upsampling, and the
+    BF16 add, in their 16-byte vector forms (c % 8 == 0) and their scalar forms
+    (c = 12), against the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(c + 5)
+    full = (2, 4, 6, 10)
+    s = R.bf16_round(rng.standard_normal((2, 2, 3, 5, c)))
+    pa = torch.empty(full + (c,), dtype=torch.int16, device="cuda")
+    U.avgpool(bf(torch, s), pa, full, c, adjoint=True)
+    assert np.array_equal(from_bf(pa), R.bf16_round(R.avgpool2_adjoint(s)))
+    fine = R.bf16_round(rng.standard_normal(full + (c + 8,)))
+    up_adj = torch.empty((2, 2, 3, 5, c), dtype=torch.int16, device="cuda")
+    U.upsample(bf(torch, fine), up_adj, (2, 2, 3, 5), c, c + 8, 8, adjoint=True)
+    assert rel(from_bf(up_adj), R.upsample2_adjoint(fine[..., 8:])) < 4e-3
+    a = R.bf16_round(rng.standard_normal(8 * 1001 if c == 32 else 1001))
+    b = R.bf16_round(rng.standard_normal(a.shape))
+    y = torch.empty(a.shape, dtype=torch.int16, device="cuda")
+    a_d, b_d = bf(torch, a), bf(torch, b)  # alive until the kernel has run
+    U.add(a_d, b_d, y)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_bf(y), R.bf16_round((a.astype(np.float32) + b.astype(np.float32))))
 
 
 def test_pool_upsample_copy_head_mse_adam(torch_cuda):
