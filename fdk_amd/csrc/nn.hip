@@ -1140,7 +1140,8 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_c1_kernel(Wgrad a) {
 // with the 3 x 3 x 3 input window in registers (9 new values per voxel, the
 // unrolled walk renames the rest) and its 27 sums, one dY value per voxel
 // (32 channels of one voxel = 64 contiguous bytes per 32 lanes).  Each block
-// adds its sums with 27 x 32 FP32 atomics at the end.  W % 64, H % 4, D % 2.
+// reduces its rows' sums in LDS and adds them with 27 x 32 FP32 atomics at the
+// end.  W % 64, H % 4, D % 2.
 constexpr int kC1X = 64;  // tile row length
 __global__ __launch_bounds__(256) void conv3d_wgrad_c1t_kernel(Wgrad a, int tiles_x, int tiles_y,
                                                                int ntiles) {
@@ -1185,8 +1186,14 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_c1t_kernel(Wgrad a, int tile
         }
     }
   }
+  // the 8 tile rows' sums meet in LDS first: 27 x 32 global atomics per block
+  __shared__ float red[27 * 32];
+  for (int i = t; i < 27 * 32; i += 256) red[i] = 0.f;
+  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 27; ++k) atomicAdd(&a.dw[co * 32 + k], acc[k]);
+  for (int k = 0; k < 27; ++k) atomicAdd(&red[k * 32 + co], acc[k]);
+  __syncthreads();
+  for (int i = t; i < 27 * 32; i += 256) atomicAdd(&a.dw[(i & 31) * 32 + (i >> 5)], red[i]);
 }
 
 // dW of a [27][cout][cin] forward weight -> backward-data weight
@@ -1478,42 +1485,42 @@ __global__ void head_bwd_kernel(const float* dout, const u16* h, const float* w,
   if (threadIdx.x == 0) atomicAdd(db, red[c]);
 }
 
-// head_bwd_kernel for C channels with 16-byte vectors: one voxel per thread,
-// its C BF16 features as C / 8 16-byte loads and its feature gradients as C / 8
-// 16-byte stores; the weight-gradient sums go through LDS atomics per block
-// and one global atomic per channel and block.
+// head_bwd_kernel for C channels with 16-byte vectors: one 16-byte chunk (8
+// features of one voxel) per thread, consecutive threads on consecutive
+// chunks; the weight-gradient sums go through LDS atomics per block and one
+// global atomic per channel and block.
 template <int C>
 __global__ __launch_bounds__(256) void head_bwd_v8_kernel(const float* dout, const uint4* h,
                                                           const float* w, uint4* dh, float* dw,
                                                           float* db, long long nvox) {
+  constexpr int G = C / 8;  // 16-byte chunks per voxel; thread i: chunk i % G of voxel i / G
   __shared__ float red[C + 1];
   for (int i = threadIdx.x; i < C + 1; i += 256) red[i] = 0.f;
   __syncthreads();
-  float acc[C];
+  const int q = threadIdx.x % G;  // fixed: 256 and the grid stride are multiples of G
+  float wq[8], acc[8];
 #pragma unroll
-  for (int ch = 0; ch < C; ++ch) acc[ch] = 0.f;
+  for (int e = 0; e < 8; ++e) {
+    wq[e] = w[8 * q + e];
+    acc[e] = 0.f;
+  }
   float accb = 0.f;
-  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long long)gridDim.x * 256) {
-    const float g = dout[v];
-    accb += g;
+  const long long nch = nvox * G;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nch; i += (long long)gridDim.x * 256) {
+    const float g = dout[i / G];
+    if (q == 0) accb += g;
+    float f[8], o[8];
+    unpack8(h[i], f);
 #pragma unroll
-    for (int q = 0; q < C / 8; ++q) {
-      const uint4 hv = h[v * (C / 8) + q];
-      const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
-      unsigned o[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int ch = 8 * q + 2 * e;
-        acc[ch] = fmaf(g, __uint_as_float(hw[e] << 16), acc[ch]);
-        acc[ch + 1] = fmaf(g, __uint_as_float(hw[e] & 0xFFFF0000u), acc[ch + 1]);
-        o[e] = (unsigned)f2bf(g * w[ch]) | ((unsigned)f2bf(g * w[ch + 1]) << 16);
-      }
-      dh[v * (C / 8) + q] = make_uint4(o[0], o[1], o[2], o[3]);
+    for (int e = 0; e < 8; ++e) {
+      acc[e] = fmaf(g, f[e], acc[e]);
+      o[e] = g * wq[e];
     }
+    dh[i] = pack8(o);
   }
 #pragma unroll
-  for (int ch = 0; ch < C; ++ch) atomicAdd(&red[ch], acc[ch]);
-  atomicAdd(&red[C], accb);
+  for (int e = 0; e < 8; ++e) atomicAdd(&red[8 * q + e], acc[e]);
+  if (q == 0) atomicAdd(&red[C], accb);
   __syncthreads();
   for (int i = threadIdx.x; i < C; i += 256) atomicAdd(&dw[i], red[i]);
   if (threadIdx.x == 0) atomicAdd(db, red[C]);
@@ -2036,7 +2043,7 @@ extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int
     FDK_REQUIRE(cout == 32, "the cin == 1 weight gradient takes cout == 32");
     if (wd % kC1X == 0 && h % 4 == 0 && d % 2 == 0 && aligned16(dy)) {
       const int tiles_x = wd / kC1X, tiles_y = h / 4, ntiles = tiles_x * tiles_y * (n * d / 2);
-      hipLaunchKernelGGL(conv3d_wgrad_c1t_kernel, dim3(std::min(ntiles, 256)), dim3(256), 0, st, a,
+      hipLaunchKernelGGL(conv3d_wgrad_c1t_kernel, dim3(std::min(ntiles, 2048)), dim3(256), 0, st, a,
                          tiles_x, tiles_y, ntiles);
     } else {
       hipLaunchKernelGGL(conv3d_wgrad_c1_kernel, dim3(2048), dim3(256), 0, st, a);
