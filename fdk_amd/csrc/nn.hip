@@ -1746,6 +1746,73 @@ __global__ __launch_bounds__(256) void head_shuffle_bwd_kernel(const float* dout
   if (t < K) atomicAdd(&db[t], accb);
 }
 
+// head_shuffle_bwd_kernel<4, 32> with every thread on the feature gradients:
+// per 64-voxel tile the block stages the 64 output gradients (16 rows of 4 fine
+// samples each) and the 32 features of every voxel in LDS, then thread t
+// computes the 8 feature gradients 8 (t % 4) .. + 7 of voxel t / 4 (64 FMAs
+// each, k ascending as in the one-thread-per-voxel form, so the same values)
+// before the weight-gradient sums of head_shuffle_bwd_kernel.
+template <int C>
+__global__ __launch_bounds__(256) void head_shuffle4_bwd_kernel(const float* dout, const uint4* h,
+                                                                const float* w, uint4* dh,
+                                                                float* dw, float* db, int hc,
+                                                                int wc, long long nvox) {
+  constexpr int F = 4, K = 64, TV = 64, E = K / 8, G = C / 8;
+  static_assert(8 * C == 256 && TV * G == 256, "8 k-groups of C threads; one chunk per thread");
+  __shared__ float ws[K * C];
+  __shared__ float gs[TV][K + 1];
+  __shared__ float hs[TV][C + 1];
+  for (int i = threadIdx.x; i < K * C; i += 256) ws[i] = w[i];
+  const int t = threadIdx.x, kg = t / C, cc = t % C;
+  const int jv = t / G, q = t % G;  // feature-gradient phase: voxel jv, channel chunk q
+  float acc[E], accb = 0.f;
+#pragma unroll
+  for (int e = 0; e < E; ++e) acc[e] = 0.f;
+  for (long long tile = (long long)blockIdx.x * TV; tile < nvox; tile += (long long)gridDim.x * TV) {
+    __syncthreads();  // the previous tile's reads are done (and ws is written)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // rows r = t % 16 of voxels t / 16 + 16 i
+      const int j = (t >> 4) + 16 * i, r = t & 15;
+      const long long v = tile + j;
+      float row[F] = {0.f, 0.f, 0.f, 0.f};
+      if (v < nvox) {
+        const S2dMap mp = s2d_map<F>(v, hc, wc);
+        row_load<F>(dout + mp.o0 + (r / F) * mp.HW + (r % F) * mp.W, row);
+      }
+#pragma unroll
+      for (int e = 0; e < F; ++e) gs[j][r * F + e] = row[e];
+    }
+    {
+      const long long v = tile + jv;
+      float f[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (v < nvox) unpack8(h[v * G + q], f);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) hs[jv][8 * q + e] = f[e];
+    }
+    __syncthreads();
+    {
+      float d8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+      for (int k = 0; k < K; ++k) {
+        const float g = gs[jv][k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) d8[e] = fmaf(ws[k * C + 8 * q + e], g, d8[e]);
+      }
+      if (tile + jv < nvox) dh[(tile + jv) * G + q] = pack8(d8);
+    }
+    for (int j = 0; j < TV; ++j) {
+      const float hj = hs[j][cc];
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = fmaf(gs[j][kg + 8 * e], hj, acc[e]);
+    }
+    if (t < K)
+      for (int j = 0; j < TV; ++j) accb += gs[j][t];
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) atomicAdd(&dw[(kg + 8 * e) * C + cc], acc[e]);
+  if (t < K) atomicAdd(&db[t], accb);
+}
+
 // Mean-squared error: loss_sum += sum (out - t)^2, dout = 2 (out - t) / nvox.
 __global__ void mse_kernel(const float* out, const float* t, float* dout, float* loss_sum,
                            long long nvox) {
@@ -2297,7 +2364,7 @@ extern "C" int fdk_nn_head_shuffle_bwd(const float* dout, const void* h, const f
                        dim3(256), 0, st, dout, static_cast<const uint4*>(h), w,
                        static_cast<uint4*>(dh), dw, db, hh / 2, ww / 2, nvox);
   else
-    hipLaunchKernelGGL((head_shuffle_bwd_kernel<4, 32>), dim3(std::min<long long>(1024, (nvox + 63) / 64)),
+    hipLaunchKernelGGL((head_shuffle4_bwd_kernel<32>), dim3(std::min<long long>(1024, (nvox + 63) / 64)),
                        dim3(256), 0, st, dout, static_cast<const uint4*>(h), w,
                        static_cast<uint4*>(dh), dw, db, hh / 4, ww / 4, nvox);
   FDK_HIP(hipGetLastError());
