@@ -900,10 +900,14 @@ __global__ __launch_bounds__(256) void conv3d_fwd_c1r_kernel(Conv a) {
 constexpr int kWgradRows = 64;
 
 struct Wgrad {
-  const u16* x;   // [n][d][h][w][cin]
+  const u16* x;   // [n][d][h][w][cin], or [n][d][h][w][cin - cup] with `up`
   const u16* dy;  // [n][d][h][w][cout]
   float* dw;      // [27][cout][cin], accumulated with atomics
   int n, d, h, w, cin, cout;
+  // Decoder form (conv3d_wgrad_tr_kernel only): the input is [nearest 2x
+  // upsampling of up ([n][d/2][h/2][w/2][cup]), x], read in place.
+  const u16* up = nullptr;
+  int cup = 0;
 };
 
 template <int MAXT>
@@ -1031,8 +1035,9 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_tr_kernel(Wgrad g, int tiles
   const int ncoc = g.cout / 32;
   const int co0 = (blockIdx.y % ncoc) * 32, ci0 = (blockIdx.y / ncoc) * 32;
   const int ct = wave & 1, it = wave >> 1;
-  // the forward kernel's tile and halo helpers, on the input tensor
-  Conv a{g.x, nullptr, nullptr, nullptr, nullptr, g.n, g.d, g.h, g.w, g.cin, 32, 0, 0};
+  // the forward kernel's tile and halo helpers, on the input tensor (or the
+  // decoder input's two tensors)
+  Conv a{g.x, nullptr, nullptr, nullptr, nullptr, g.n, g.d, g.h, g.w, g.cin, 32, 0, 0, g.up, g.cup};
   const unsigned char* hb = reinterpret_cast<const unsigned char*>(halo);
   const unsigned char* db = reinterpret_cast<const unsigned char*>(dyt);
   // per-lane byte offsets of the transposing reads: voxel x0 + sl / 4 (+ 4 for
@@ -2100,6 +2105,40 @@ extern "C" int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, con
   return launch_fwd4(a, static_cast<hipStream_t>(stream));
 }
 
+// Shapes conv3d_wgrad_tr_kernel takes: 32-channel chunks of cin (and of cup),
+// cout % 32, H % 4, even D, 16-byte vectors, 32-bit halo offsets.
+static bool wgrad_tr_ok(const Wgrad& a) {
+  const int cx = a.cin - a.cup;
+  return conv_kernel_choice() >= 4 && a.cin % 32 == 0 && a.cup % 32 == 0 && a.cout % 32 == 0 &&
+         a.h % 4 == 0 && a.d % 2 == 0 && a.w % 32 == 0 && aligned16(a.x) && aligned16(a.dy) &&
+         (!a.up || aligned16(a.up)) && (4LL * a.h * a.w + 64) * std::max(cx, a.cup) < (1LL << 31);
+}
+
+static int launch_wgrad_tr(const Wgrad& a, hipStream_t st) {
+  const int tiles_x = a.w / 32, tiles_y = a.h / 4, vt = tiles_x * tiles_y * (a.n * a.d / 2);
+  const int chunks = (a.cout / 32) * (a.cin / 32);
+  // about 1024 blocks: every CU busy several times over, few partial sums to add
+  const int gx = std::max(1, std::min(vt, 1024 / chunks));
+  hipLaunchKernelGGL(conv3d_wgrad_tr_kernel, dim3(gx, chunks), dim3(256), 0, st, a, tiles_x, tiles_y, vt);
+  FDK_HIP(hipGetLastError());
+  return FDK_OK;
+}
+
+extern "C" int fdk_nn_conv3d_wgrad_cat(const void* up, int cup, const void* x, const void* dy,
+                                       float* dw, int n, int d, int h, int wd, int cin, int cout,
+                                       void* stream) {
+  FDK_REQUIRE(up && x && dy && dw, "NULL pointer");
+  FDK_REQUIRE(n > 0 && d > 0 && h > 0 && wd > 0, "bad tensor size");
+  FDK_REQUIRE(cup > 0 && cup < cin && cup % 32 == 0 && cin % 32 == 0,
+              "conv3d_wgrad_cat needs 0 < cup < cin, both multiples of 32 (got %d, %d)", cup, cin);
+  Wgrad a{static_cast<const u16*>(x), static_cast<const u16*>(dy), dw, n, d, h, wd, cin, cout,
+          static_cast<const u16*>(up), cup};
+  if (!wgrad_tr_ok(a))
+    return fdk::fail(FDK_ERR_UNSUPPORTED, "conv3d_wgrad_cat: cout %% 32, W %% 32, H %% 4, even D, "
+                     "16-byte aligned tensors and 32-bit halo offsets required");
+  return launch_wgrad_tr(a, static_cast<hipStream_t>(stream));
+}
+
 extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h,
                                    int wd, int cin, int cout, void* stream) {
   FDK_REQUIRE(x && dy && dw, "NULL pointer");
@@ -2119,16 +2158,7 @@ extern "C" int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int
     return FDK_OK;
   }
   FDK_REQUIRE(cin % 16 == 0 && wd % 32 == 0, "cin %% 16 and W %% 32 must be 0");
-  if (conv_kernel_choice() >= 4 && cin % 32 == 0 && cout % 32 == 0 && h % 4 == 0 && d % 2 == 0 &&
-      aligned16(x) && aligned16(dy) && (4LL * h * wd + 64) * cin < (1LL << 31)) {
-    const int tiles_x = wd / 32, tiles_y = h / 4, vt = tiles_x * tiles_y * (n * d / 2);
-    const int chunks = (cout / 32) * (cin / 32);
-    // about 1024 blocks: every CU busy several times over, few partial sums to add
-    const int gx = std::max(1, std::min(vt, 1024 / chunks));
-    hipLaunchKernelGGL(conv3d_wgrad_tr_kernel, dim3(gx, chunks), dim3(256), 0, st, a, tiles_x, tiles_y, vt);
-    FDK_HIP(hipGetLastError());
-    return FDK_OK;
-  }
+  if (wgrad_tr_ok(a)) return launch_wgrad_tr(a, st);
   const int ntiles = (cout / 16) * (cin / 16);
   const int per_wave = (ntiles + 3) / 4;
   const size_t lds = (size_t)(cout + cin) * (wd + 8) * sizeof(u16);

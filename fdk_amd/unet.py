@@ -48,6 +48,7 @@ def _sig():
     L.fdk_nn_conv3d_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
     L.fdk_nn_conv3d_fwd_cat.argtypes = [vp, i, vp, vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
     L.fdk_nn_conv3d_wgrad.argtypes = [vp, vp, vp, i, i, i, i, i, i, vp]
+    L.fdk_nn_conv3d_wgrad_cat.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, i, vp]
     L.fdk_nn_flip_weights.argtypes = [vp, vp, i, i, vp]
     L.fdk_nn_bias_grad.argtypes = [vp, vp, i64, i, vp]
     L.fdk_nn_relu_bwd.argtypes = [vp, vp, vp, i64, i, vp]
@@ -106,6 +107,14 @@ def conv3d_cat(up, cup, x, w, bias, y, shape, cin, cout, relu=True):
 def conv3d_wgrad(x, dy, dw, shape, cin, cout):
     n, d, h, wd = shape
     _check(L().fdk_nn_conv3d_wgrad(_p(x), _p(dy), _p(dw), n, d, h, wd, cin, cout, _st()), "wgrad")
+
+
+def conv3d_wgrad_cat(up, cup, x, dy, dw, shape, cin, cout):
+    """conv3d_wgrad of the input [nearest 2x upsampling of up (cup channels), x]
+    read without forming it; shape is the full-resolution (n, d, h, w)."""
+    n, d, h, wd = shape
+    _check(L().fdk_nn_conv3d_wgrad_cat(_p(up), int(cup), _p(x), _p(dy), _p(dw), n, d, h, wd, cin,
+                                       cout, _st()), "wgrad_cat")
 
 
 def flip_weights(w, wb, cin, cout):
@@ -243,11 +252,16 @@ class Conv3:
         y = torch.empty(shape + (self.cout,), dtype=I16, device=x.device)
         return conv3d_cat(up, cup, x, self.wbf, self.b, y, shape, self.cin, self.cout, self.relu)
 
-    def backward(self, dy, x, y, shape, need_dx=True):
-        """dy: gradient at the (post-ReLU) output y; x: the layer input."""
+    def backward(self, dy, x, y, shape, need_dx=True, cat=None):
+        """dy: gradient at the (post-ReLU) output y; x: the layer input, or with
+        cat = (up, cup) the skip half of the input [upsample2(up), x] that
+        forward_cat read."""
         torch = _torch()
         dz = relu_bwd(dy, y, torch.empty_like(dy), self.relu) if self.relu else dy
-        conv3d_wgrad(x, dz, self.gw, shape, self.cin, self.cout)
+        if cat is None:
+            conv3d_wgrad(x, dz, self.gw, shape, self.cin, self.cout)
+        else:
+            conv3d_wgrad_cat(cat[0], cat[1], x, dz, self.gw, shape, self.cin, self.cout)
         bias_grad(dz, self.gb, int(np.prod(shape)), self.cout)
         if not need_dx:
             return None
@@ -307,6 +321,9 @@ class UNet3D:
                        torch.zeros_like(self.hb), torch.zeros_like(self.hb)]
         self.step_count = 0
         self.cache = None
+        # decoder inputs read in place (forward and weight gradients); False
+        # forms the concatenations instead (same values)
+        self.fuse_decoder = True
 
     @property
     def convs(self):
@@ -359,11 +376,13 @@ class UNet3D:
         bb = self.bb.forward(ba, s3_)
         hold(ba=ba)
         del ba
-        # Decoder inputs [upsample2(coarse), skip]: formed only when backward()
-        # needs them (keep) or the fused kernel does not serve the layer.
-        fuse = not keep and self.d2a.cat_ok(c3) and self.d1a.cat_ok(c2)
+        # Decoder inputs [upsample2(coarse), skip]: read in place by the fused
+        # kernels (backward() then reads them in place too), formed only when
+        # those kernels do not serve the layer.
+        fuse = self.fuse_decoder and self.d2a.cat_ok(c3) and self.d1a.cat_ok(c2)
         if fuse:
             d2a = self.d2a.forward_cat(bb, c3, s2, s2_)
+            hold(bb=bb, s2=s2)
             del bb, s2
         else:
             cat2 = bf(s2_, c3 + c2)
@@ -379,6 +398,7 @@ class UNet3D:
         del d2a
         if fuse:
             d1a = self.d1a.forward_cat(d2, c2, s1, s1_)
+            hold(d2=d2, s1=s1)
             del d2, s1
         else:
             cat1 = bf(s1_, c2 + c1)
@@ -425,13 +445,19 @@ class UNet3D:
             _check(L().fdk_nn_head_bwd(_p(dout), _p(k["d1"]), _p(self.hw), _p(dd1), _p(self.ghw),
                                        _p(self.ghb), dout.numel(), c1, _st()), "head_bwd")
         dd1a = self.d1b.backward(dd1, k["d1a"], k["d1"], s1_)
-        dcat1 = self.d1a.backward(dd1a, k["cat1"], k["d1a"], s1_)
+        if "cat1" in k:
+            dcat1 = self.d1a.backward(dd1a, k["cat1"], k["d1a"], s1_)
+        else:  # the fused forward: the weight gradient reads [upsample2(d2), s1] in place
+            dcat1 = self.d1a.backward(dd1a, k["s1"], k["d1a"], s1_, cat=(k["d2"], c2))
         dd2 = upsample(dcat1, torch.empty(s2_ + (c2,), dtype=I16, device=dev), s2_, c2, c2 + c1,
                        0, adjoint=True)
         ds1 = torch.empty(s1_ + (c1,), dtype=I16, device=dev)
         channel_copy(dcat1, c2 + c1, c2, ds1, c1, 0, c1, n1)
         dd2a = self.d2b.backward(dd2, k["d2a"], k["d2"], s2_)
-        dcat2 = self.d2a.backward(dd2a, k["cat2"], k["d2a"], s2_)
+        if "cat2" in k:
+            dcat2 = self.d2a.backward(dd2a, k["cat2"], k["d2a"], s2_)
+        else:
+            dcat2 = self.d2a.backward(dd2a, k["s2"], k["d2a"], s2_, cat=(k["bb"], c3))
         dbb = upsample(dcat2, torch.empty(s3_ + (c3,), dtype=I16, device=dev), s3_, c3, c3 + c2,
                        0, adjoint=True)
         ds2 = torch.empty(s2_ + (c2,), dtype=I16, device=dev)

@@ -325,6 +325,15 @@ int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, const void* w,
  * cin == 1 accumulates into [cout][32]. */
 int fdk_nn_conv3d_wgrad(const void* x, const void* dy, float* dw, int n, int d, int h, int w_,
                         int cin, int cout, void* stream);
+/* The same weight gradient for a decoder convolution whose input is the
+ * channel concatenation [nearest 2x upsampling of up, x] (up:
+ * [n][d/2][h/2][w/2][cup], x: [n][d][h][w][cin - cup]), read from the two
+ * tensors without forming it (the input of fdk_nn_conv3d_fwd_cat).  Needs cin,
+ * cup and cout multiples of 32, W % 32, H % 4, even d, 16-byte aligned
+ * tensors; FDK_ERR_UNSUPPORTED otherwise.  No counterpart in the reference (the
+ * learned path's training). */
+int fdk_nn_conv3d_wgrad_cat(const void* up, int cup, const void* x, const void* dy, float* dw,
+                            int n, int d, int h, int w_, int cin, int cout, void* stream);
 /* Backward-data weights: wb[t][ci][co] = w[26 - t][co][ci]; the input gradient
  * is then fdk_nn_conv3d_fwd(dy, wb, ..., cin = cout, cout = cin). */
 int fdk_nn_flip_weights(const void* w, void* wb, int cin, int cout, void* stream);

@@ -135,9 +135,38 @@ def test_unet_inference_fused_decoder_matches_formed(torch_cuda):
         net.hw.mul_(300.0)
     x = torch.as_tensor(np.random.default_rng(9).standard_normal((1, 8, 16, 128)).astype(np.float32)).cuda()
     fused = net.forward(x).clone()
+    kept = net.forward(x, keep=True).clone()
+    net.fuse_decoder = False
     formed = net.forward(x, keep=True)
     torch.cuda.synchronize()
     assert torch.equal(fused, formed)
+    assert torch.equal(kept, formed)
+
+
+@pytest.mark.parametrize("s2d", [1, 4])
+def test_unet_gradients_with_fused_decoder_inputs(torch_cuda, s2d):
+    """Training with the decoder inputs read in place (forward and weight
+    gradients through fdk_nn_conv3d_wgrad_cat) gives the gradients of the
+    formed-concatenation path: the same kernels on the same values, up to the
+    order of the blocks' FP32 atomics."""
+    torch = torch_cuda
+    shape = (1, 8, 16, 128) if s2d == 1 else (1, 16, 64, 512)
+    x = torch.as_tensor(np.random.default_rng(21).standard_normal(shape).astype(np.float32)).cuda()
+    t = torch.as_tensor(np.random.default_rng(22).standard_normal(shape).astype(np.float32)).cuda()
+    grads = []
+    for fuse in (True, False):
+        net = U.UNet3D(base=32, seed=5, s2d=s2d)
+        net.fuse_decoder = fuse
+        net.zero_grad()
+        out = net.forward(x, keep=True)
+        dout = torch.empty_like(out)
+        loss = torch.zeros(1, device="cuda")
+        U.mse(out, t, dout, loss)
+        net.backward(dout)
+        torch.cuda.synchronize()
+        grads.append([cv.gw.cpu().numpy().copy() for cv in net.convs])
+    for a, b in zip(*grads):
+        assert rel(a, b) < 1e-5
 
 
 @pytest.mark.parametrize("shape,cout", [((1, 4, 8, 64), 32), ((2, 3, 8, 96), 32),
