@@ -1050,7 +1050,7 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_tr_kernel(Wgrad g, int tiles
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const Fwd4Tile f = fwd4_tile(a, tile, tiles_x, tiles_y);
     uint4 pre[kHaloPer];
-    fwd4_fetch(pre, a, f, ci0);
+    fwd5_fetch<2, 4>(pre, a, f, ci0);
     // dY chunk: slot s = t + 256 j is tile row s / 128 (plane (s / 128) / 4, row
     // (s / 128) % 4), voxel (s % 128) / 4, 8-channel group s % 4; a tile row
     // is 32 voxels x 64 bytes, so the slots of one row are contiguous
@@ -2106,12 +2106,14 @@ extern "C" int fdk_nn_conv3d_fwd_cat(const void* up, int cup, const void* x, con
 }
 
 // Shapes conv3d_wgrad_tr_kernel takes: 32-channel chunks of cin (and of cup),
-// cout % 32, H % 4, even D, 16-byte vectors, 32-bit halo offsets.
+// cout % 32, H % 4, even D, 16-byte vectors, its halo resource (4 planes + 2
+// rows) below 2^32 - 16 bytes.
 static bool wgrad_tr_ok(const Wgrad& a) {
   const int cx = a.cin - a.cup;
   return conv_kernel_choice() >= 4 && a.cin % 32 == 0 && a.cup % 32 == 0 && a.cout % 32 == 0 &&
          a.h % 4 == 0 && a.d % 2 == 0 && a.w % 32 == 0 && aligned16(a.x) && aligned16(a.dy) &&
-         (!a.up || aligned16(a.up)) && (4LL * a.h * a.w + 64) * std::max(cx, a.cup) < (1LL << 31);
+         (!a.up || aligned16(a.up)) &&
+         (4LL * a.h * a.w + 2LL * a.w) * std::max(cx, a.cup) < (1LL << 31) - 8;
 }
 
 static int launch_wgrad_tr(const Wgrad& a, hipStream_t st) {
