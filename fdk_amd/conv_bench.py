@@ -10,7 +10,8 @@ TFLOP/s (2 * 27 * cin * cout FLOP per output voxel):
 * forward (fdk_nn_conv3d_fwd, or fdk_nn_conv3d_fwd_cat with --cup: the
   decoder input [2x upsampled coarse tensor of cup channels, skip tensor of
   cin - cup channels], read in place);
-* with --wgrad, the weight gradient (fdk_nn_conv3d_wgrad) of the same layer.
+* with --wgrad, the weight gradient of the same layer (fdk_nn_conv3d_wgrad, or
+  fdk_nn_conv3d_wgrad_cat with --cup: the decoder input read in place).
 
 The layer kernels are the library's; nothing here falls back to torch.
 """
@@ -73,12 +74,14 @@ def main(argv=None) -> None:
     ms = timed(fwd)
     print(f"conv3d {cin}->{cout} {shape} {form}: {ms:.3f} ms, {flop / ms / 1e9:.1f} TFLOP/s", flush=True)
     if args.wgrad:
-        if cup:
-            raise SystemExit("--wgrad takes the plain form (no --cup)")
         dy = bf16(*shape, cout)
         dw = torch.zeros((27, cout, cin), device="cuda")
-        ms = timed(lambda: U.conv3d_wgrad(x, dy, dw, shape, cin, cout))
-        print(f"conv3d {cin}->{cout} {shape} weight gradient: {ms:.3f} ms, "
+        if cup:
+            ms = timed(lambda: U.conv3d_wgrad_cat(up, cup, skip, dy, dw, shape, cin, cout))
+        else:
+            ms = timed(lambda: U.conv3d_wgrad(x, dy, dw, shape, cin, cout))
+        form = f"weight gradient{' (decoder input, cup %d)' % cup if cup else ''}"
+        print(f"conv3d {cin}->{cout} {shape} {form}: {ms:.3f} ms, "
               f"{flop / ms / 1e9:.1f} TFLOP/s", flush=True)
 
 
